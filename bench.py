@@ -1,0 +1,231 @@
+"""Benchmark: native FP64 MFMA DGEMM at M=N=K=8192 on one MI355X.
+
+A step is one C = A.B call (f64, row-major, entries normal(0,1) from the
+seeded device generator) with A, B and C resident in HBM.  W untimed warm-up
+steps, then K steps timed with HIP events on the stream the kernels run on,
+bracketed by device synchronisation.  One JSON line is printed.
+
+Besides the headline the line carries:
+  roofline      - the DGEMM kernel's achieved FLOP/s against the FP64 matrix
+                  peak (78.6 TFLOP/s), plus HBM traffic per launch when the
+                  rocprofv3 counter summary for it is present under profiles/;
+  cpu_baseline  - the oracle's restatement of ferray's CPU GEMM (Goto 5-loop,
+                  AVX2 4x8 kernel, min(nproc, 8) workers) timed on this host
+                  on a bounded 4096^3 f64 sample;
+  error         - normwise error of the 8192 result against a compensated
+                  (double-double) dot product on sampled rows and columns;
+  sweep         - the other dtypes of the GEMM family at their config sizes
+                  (f32 8192, c64/c32 4096, i8 8192, i16 4096, bf16 8192),
+                  one timed number each (--no-sweep skips them).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("GEMM TFLOP/s at 8192 (f64 native vs split-precision, % peak); "
+          "serving p99 ms")
+PEAK = {  # MI355X published dense peaks, TFLOP/s (TOP/s for int8)
+    "f64": 78.6, "f32": 157.3, "bf16": 2500.0, "f16": 2500.0, "i8": 5000.0,
+}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_steps(fa, launch, steps: int, warmup: int) -> float:
+    """ms per step: events on the library's stream around `steps` calls."""
+    for _ in range(warmup):
+        launch()
+    fa.device_sync()
+    e0, e1 = fa.Event(), fa.Event()
+    e0.record()
+    for _ in range(steps):
+        launch()
+    e1.record()
+    ms = e0.elapsed_ms(e1) / steps
+    fa.device_sync()
+    return ms
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch from profiles/<name>_traffic.json, if recorded."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def dgemm_error(fa, dA, dB, dC, n: int, samples: int = 16):
+    import oracle
+    a = dA.download(np.float64, n * n)
+    b = dB.download(np.float64, n * n)
+    rng = np.random.default_rng(123)
+    rows = np.sort(rng.choice(n, samples, replace=False))
+    cols = np.sort(rng.choice(n, samples, replace=False))
+    hi, lo = oracle.dd_gemm_f64_sampled(n, n, n, a, b, rows, cols)
+    c_rows = np.stack([dC.download(np.float64, n, offset_bytes=int(r) * n * 8) for r in rows])
+    got = c_rows[:, cols]
+    d = (got - hi) - lo
+    na, nb = np.linalg.norm(a), np.linalg.norm(b)
+    # scale the sampled error to the full matrix (mean square per element)
+    frac = (n * n) / (samples * samples)
+    err_f = np.sqrt(np.sum(d * d) * frac)
+    c_f = np.sqrt(np.sum(hi * hi) * frac)
+    return {"normwise_ab": float(err_f / (na * nb)), "rel_c": float(err_f / c_f),
+            "max_abs": float(np.max(np.abs(d))),
+            "sample": f"{samples} rows x {samples} cols vs compensated dot products"}
+
+
+def cpu_baseline(n: int = 4096, reps: int = 3):
+    import oracle
+    a = oracle.uniform_f64(n * n, 1)
+    b = oracle.uniform_f64(n * n, 2)
+    c = np.zeros(n * n)
+    workers = oracle.gemm_f64_goto(n, n, n, 1.0, a, b, 0.0, c)   # warm-up
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.gemm_f64_goto(n, n, n, 1.0, a, b, 0.0, c)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": 2.0 * n ** 3 / t / 1e12, "unit": "TFLOP/s", "cores": int(workers),
+            "kind": "port",
+            "sample": (f"ferray CPU DGEMM restated in C (Goto 5-loop MC/KC/NC=64/256/512, AVX2 4x8, "
+                       f"pick_grid, min(nproc,8) workers); f64 {n}^3 uniform[-1,1), median of "
+                       f"{reps} after 1 warm-up, {t * 1e3:.0f} ms per call; avx2="
+                       f"{oracle.cpu_has_avx2_fma()}")}
+
+
+def run_sweep(fa, steps: int, warmup: int):
+    out = {}
+    cases = [("sgemm_f32_8192", "f32", 8192), ("zgemm_c64_4096", "c64", 4096),
+             ("cgemm_c32_4096", "c32", 4096), ("igemm_i8_8192", "i8", 8192),
+             ("igemm_i16_4096", "i16", 4096), ("hgemm_bf16_8192", "bf16", 8192)]
+    for name, dt, n in cases:
+        esz = {"f32": 4, "c64": 16, "c32": 8, "i8": 1, "i16": 2, "bf16": 2}[dt]
+        csz = {"f32": 4, "c64": 16, "c32": 8, "i8": 4, "i16": 4, "bf16": 4}[dt]
+        dA, dB, dC = fa.DeviceBuffer(n * n * esz), fa.DeviceBuffer(n * n * esz), \
+            fa.DeviceBuffer(n * n * csz)
+        if dt in ("f32", "c32"):
+            cnt = n * n * (2 if dt == "c32" else 1)
+            fa.fill(dA, cnt, fa.FILL_F32_NORMAL, 11); fa.fill(dB, cnt, fa.FILL_F32_NORMAL, 12)
+        elif dt == "c64":
+            fa.fill(dA, 2 * n * n, fa.FILL_F64_NORMAL, 11); fa.fill(dB, 2 * n * n, fa.FILL_F64_NORMAL, 12)
+        elif dt == "i8":
+            fa.fill(dA, n * n, fa.FILL_I8, 11, -128, 127); fa.fill(dB, n * n, fa.FILL_I8, 12, -128, 127)
+        elif dt == "i16":
+            fa.fill(dA, n * n, fa.FILL_I16, 11, -32768, 32767)
+            fa.fill(dB, n * n, fa.FILL_I16, 12, -32768, 32767)
+        else:  # bf16: random bit patterns with a bounded exponent
+            fa.fill(dA, n * n, fa.FILL_I16, 11, 0x3c00, 0x3fff)
+            fa.fill(dB, n * n, fa.FILL_I16, 12, 0xbc00, 0xbfff)
+        L = fa.lib()
+        fns = {
+            "f32": lambda: L.ferray_gemm_f32_dev(n, n, n, 1.0, dA.ptr, dB.ptr, 0.0, dC.ptr, None),
+            "c64": lambda: L.ferray_gemm_c64_dev(n, n, n, 1.0, 0.0, dA.ptr, dB.ptr, 0.0, 0.0, dC.ptr, None),
+            "c32": lambda: L.ferray_gemm_c32_dev(n, n, n, 1.0, 0.0, dA.ptr, dB.ptr, 0.0, 0.0, dC.ptr, None),
+            "i8": lambda: L.ferray_gemm_i8_signed_dev(n, n, n, dA.ptr, dB.ptr, dC.ptr, False, None),
+            "i16": lambda: L.ferray_gemm_i16_dev(n, n, n, dA.ptr, dB.ptr, dC.ptr, False, None),
+            "bf16": lambda: L.ferray_gemm_bf16_f32_dev(n, n, n, dA.ptr, dB.ptr, dC.ptr, False, None),
+        }
+        fn = fns[dt]
+        st = fn()
+        if st != 0:
+            out[name] = {"error": f"status {st}"}
+            continue
+        ms = time_steps(fa, fn, max(3, steps // 2), max(1, warmup))
+        flops = (8.0 if dt in ("c64", "c32") else 2.0) * n ** 3
+        rate = flops / (ms * 1e-3) / 1e12
+        peak_key = {"c64": "f64", "c32": "f32", "i16": "i8"}.get(dt, dt)
+        peak = PEAK[peak_key] / (4.0 if dt == "i16" else 1.0)  # i16 = 4 int8 products
+        out[name] = {"ms": round(ms, 4), "rate": round(rate, 2),
+                     "unit": "TOP/s" if dt in ("i8", "i16") else "TFLOP/s",
+                     "frac_of_peak": round(rate / peak, 4)}
+        log(f"{name}: {ms:.3f} ms  {rate:.1f}  ({rate / peak * 100:.1f}% of peak)")
+        for buf in (dA, dB, dC):
+            buf.free()
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=8192)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the GEMM path runs on a single MI355X); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        return 2
+
+    import ferray_amd as fa
+    if not fa.gpu_supports():
+        print("no usable gfx950 device", file=sys.stderr)
+        return 1
+    n = args.size
+    log(f"DGEMM {n}^3 f64: allocating and filling operands")
+    dA, dB, dC = fa.DeviceBuffer(n * n * 8), fa.DeviceBuffer(n * n * 8), fa.DeviceBuffer(n * n * 8)
+    fa.fill(dA, n * n, fa.FILL_F64_NORMAL, 1)
+    fa.fill(dB, n * n, fa.FILL_F64_NORMAL, 2)
+    L = fa.lib()
+
+    def step():
+        st = L.ferray_gemm_f64_dev(n, n, n, 1.0, dA.ptr, dB.ptr, 0.0, dC.ptr, None)
+        if st != 0:
+            raise RuntimeError(f"ferray_gemm_f64_dev status {st}")
+
+    ms = time_steps(fa, step, args.steps, args.warmup)
+    flops = 2.0 * n ** 3
+    tflops = flops / (ms * 1e-3) / 1e12
+    log(f"DGEMM {n}^3: {ms:.3f} ms/step, {tflops:.2f} TFLOP/s "
+        f"({tflops / PEAK['f64'] * 100:.1f}% of FP64 matrix peak)")
+
+    traffic = load_traffic(f"dgemm_f64_{n}")
+    roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK["f64"],
+            "unit": "TFLOP/s", "frac": round(tflops / PEAK["f64"], 4), "traffic": traffic,
+            "kernel": "gemm_real_kernel<double,false,false>",
+            "algorithmic_work": f"2*{n}^3 = {flops:.4g} flop per launch"}
+
+    result = {
+        "metric": METRIC, "value": round(tflops, 3), "unit": "TFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded device generator, normal(0,1))",
+        "config": {"workload": f"dgemm_native_mfma_{n}", "m": n, "n": n, "k": n,
+                   "layout": "row-major", "seeds": [1, 2], "parallelism": "single GPU"},
+        "roofline": roof,
+    }
+    if not args.no_error:
+        log("sampled error against compensated dot products")
+        result["error"] = dgemm_error(fa, dA, dB, dC, n)
+    for buf in (dA, dB, dC):
+        buf.free()
+    if not args.no_sweep:
+        result["sweep"] = run_sweep(fa, args.steps, args.warmup)
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle restatement of ferray's Goto DGEMM)")
+        result["cpu_baseline"] = cpu_baseline()
+        log(f"CPU baseline: {result['cpu_baseline']['value'] * 1e3:.1f} GFLOP/s on "
+            f"{result['cpu_baseline']['cores']} threads")
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,110 @@
+"""ctypes binding of libferray_gemm.so (the C ABI in include/ferray_gemm.h).
+
+The library is loaded from the package directory.  There is no fallback: if
+the shared object is missing, every call raises instead of silently computing
+on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libferray_gemm.so")
+
+_sz = C.c_size_t
+_p = C.c_void_p
+_b = C.c_bool
+_i = C.c_int
+_d = C.c_double
+_f = C.c_float
+_u8 = C.c_uint8
+_i8 = C.c_int8
+_u64 = C.c_uint64
+_i64 = C.c_int64
+
+# name -> (restype, argtypes); every function declared in include/ferray_gemm.h
+SIGNATURES: dict[str, tuple] = {
+    "ferray_gemm_f64": (_b, [_sz, _sz, _sz, _d, _p, _p, _d, _p]),
+    "ferray_gemm_f32": (_b, [_sz, _sz, _sz, _f, _p, _p, _f, _p]),
+    "ferray_gemm_c64": (_b, [_sz, _sz, _sz, _d, _d, _p, _p, _d, _d, _p]),
+    "ferray_gemm_c32": (_b, [_sz, _sz, _sz, _f, _f, _p, _p, _f, _f, _p]),
+    "ferray_gemm_i8": (_b, [_sz, _sz, _sz, _p, _p, _p, _b]),
+    "ferray_gemm_i8_signed": (_b, [_sz, _sz, _sz, _p, _p, _p, _b]),
+    "ferray_gemm_i16": (_b, [_sz, _sz, _sz, _p, _p, _p, _b]),
+    "ferray_quantized_matmul": (_b, [_sz, _sz, _sz, _p, _u8, _f, _p, _i8, _f, _p, _b]),
+    "ferray_gemm_bf16_f32": (_b, [_sz, _sz, _sz, _p, _p, _p, _b]),
+    "ferray_gemm_f16_f32": (_b, [_sz, _sz, _sz, _p, _p, _p, _b]),
+    "ferray_gpu_available": (_b, []),
+    "ferray_gemm_last_status": (_i, []),
+    "ferray_status_string": (C.c_char_p, [_i]),
+    "ferray_gemm_f64_dev": (_i, [_sz, _sz, _sz, _d, _p, _p, _d, _p, _p]),
+    "ferray_gemm_f32_dev": (_i, [_sz, _sz, _sz, _f, _p, _p, _f, _p, _p]),
+    "ferray_gemm_c64_dev": (_i, [_sz, _sz, _sz, _d, _d, _p, _p, _d, _d, _p, _p]),
+    "ferray_gemm_c32_dev": (_i, [_sz, _sz, _sz, _f, _f, _p, _p, _f, _f, _p, _p]),
+    "ferray_gemm_i8_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
+    "ferray_gemm_i8_signed_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
+    "ferray_gemm_i16_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
+    "ferray_quantized_matmul_dev": (_i, [_sz, _sz, _sz, _p, _u8, _f, _p, _i8, _f, _p, _b, _p]),
+    "ferray_gemm_bf16_f32_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
+    "ferray_gemm_f16_f32_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
+    "ferray_dev_malloc": (_i, [_sz, C.POINTER(_p)]),
+    "ferray_dev_free": (_i, [_p]),
+    "ferray_dev_upload": (_i, [_p, _p, _sz, _p]),
+    "ferray_dev_download": (_i, [_p, _p, _sz, _p]),
+    "ferray_dev_memset": (_i, [_p, _i, _sz, _p]),
+    "ferray_stream_sync": (_i, [_p]),
+    "ferray_default_stream": (_p, []),
+    "ferray_dev_fill": (_i, [_p, _sz, _i, _u64, _i64, _i64, _p]),
+    "ferray_event_create": (_i, [C.POINTER(_p)]),
+    "ferray_event_record": (_i, [_p, _p]),
+    "ferray_event_elapsed_ms": (_i, [_p, _p, C.POINTER(_f)]),
+    "ferray_event_destroy": (_i, [_p]),
+    "ferray_device_sync": (_i, []),
+}
+
+_lock = threading.Lock()
+_lib: C.CDLL | None = None
+
+
+class FerrayError(RuntimeError):
+    """A device entry point returned a non-zero status."""
+
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        super().__init__(f"{what}: status {status} ({status_string(status)})")
+
+
+def lib() -> C.CDLL:
+    """Loads and returns the library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build it with "
+                    "`python -c 'import __graft_entry__ as g; g.build()'` or "
+                    "`python ferray_amd/build.py`")
+            handle = C.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(handle, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = handle
+    return _lib
+
+
+def status_string(status: int) -> str:
+    try:
+        s = lib().ferray_status_string(status)
+        return s.decode() if s else "?"
+    except Exception:  # pragma: no cover - only when the library is absent
+        return "?"
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        raise FerrayError(status, what)

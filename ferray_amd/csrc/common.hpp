@@ -1,0 +1,45 @@
+// Shared definitions for the gfx950 GEMM library: status codes, error
+// plumbing and the MFMA operand vector types.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/ferray_gemm.h"
+
+namespace fg {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef int i4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+#define FG_CHECK(expr)                                     \
+  do {                                                     \
+    hipError_t _e = (expr);                                \
+    if (_e != hipSuccess) return ::fg::map_hip_error(_e);  \
+  } while (0)
+
+inline int map_hip_error(hipError_t e) {
+  if (e == hipErrorOutOfMemory) return FERRAY_ERR_OOM;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return FERRAY_ERR_NO_DEVICE;
+  if (e == hipErrorInvalidValue) return FERRAY_ERR_INVALID;
+  return FERRAY_ERR_HIP;
+}
+
+// Resolves a caller stream argument: NULL means the calling thread's own
+// library stream (created on first use, default priority, non-blocking).
+hipStream_t resolve_stream(void* stream);
+
+// Grow-only device scratch owned by the library, one buffer per stream.
+// Work ordered on `stream` may use the returned memory until the next
+// call for the same stream; growing synchronises that stream first.
+int workspace(hipStream_t stream, size_t bytes, void** out);
+
+// Runtime initialisation on the calling thread (device selection).
+int ensure_device();
+
+inline size_t cdiv(size_t a, size_t b) { return (a + b - 1) / b; }
+
+}  // namespace fg

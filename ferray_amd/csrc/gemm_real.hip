@@ -1,0 +1,355 @@
+// Real and complex floating-point GEMM on CDNA4 matrix cores.
+//
+//   f64: v_mfma_f64_16x16x4_f64      f32: v_mfma_f32_16x16x4_f32
+//
+// One kernel template covers DGEMM, SGEMM, ZGEMM and CGEMM.  A complex
+// product C(m x n) = A(m x k) B(k x n) with interleaved [re, im] storage is
+// computed as the real product
+//     C_r(m x 2n) = A_r(m x 2k) . B_x(2k x 2n),
+//     B_x[2p][2j] = br   B_x[2p][2j+1] = bi
+//     B_x[2p+1][2j] = -bi  B_x[2p+1][2j+1] = br
+// where A_r is A's memory read as reals and B_x is expanded from B while the
+// tile is written to LDS.  C_r is C's interleaved memory, so there is no extra
+// HBM pass and the real flop count is exactly 8mnk.
+//
+// Block: 256 threads (4 waves, 2x2), block tile 128x128, BK = 16, each wave
+// owns a 64x64 sub-tile = 4x4 MFMA tiles of 16x16.  A and B panels go through
+// registers into double-buffered LDS (one barrier per k-step).  Tiles are
+// ordered in groups of GROUP_M rows of tiles, and the linear block id is
+// remapped so each XCD (blocks are dealt round-robin to the 8 XCDs) works on
+// a contiguous range of that order, which keeps the A and B panels a group
+// shares inside one XCD's L2.
+#include "common.hpp"
+
+namespace fg {
+
+template <typename T>
+struct Mfma16x16x4;
+template <>
+struct Mfma16x16x4<double> {
+  using acc_t = d4;
+  static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma16x16x4<float> {
+  using acc_t = f4;
+  static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// Eight consecutive elements from a 16-byte aligned address as 128-bit loads.
+template <typename T>
+__device__ __forceinline__ void load8(const T* src, T (&dst)[8]) {
+  constexpr int V = 16 / sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(V)));
+  const vec_t* s = reinterpret_cast<const vec_t*>(src);
+#pragma unroll
+  for (int q = 0; q < 8 / V; ++q) {
+    vec_t v = s[q];
+#pragma unroll
+    for (int e = 0; e < V; ++e) dst[q * V + e] = v[e];
+  }
+}
+
+template <typename T>
+struct RealArgs {
+  const T* a;
+  const T* b;
+  T* c;
+  int M, N, K;  // real dimensions (complex: N = 2n, K = 2k)
+  T alpha_re, alpha_im, beta_re, beta_im;
+  int alpha_one;  // alpha == 1 (+0i): skip the scaling
+  int beta_zero;  // beta == 0: C is written without being read
+};
+
+constexpr int RB_M = 128, RB_N = 128, RB_K = 16, R_THREADS = 256, R_GROUP_M = 8;
+
+__device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
+  int nt = tiles_m * tiles_n;
+  int bid = blockIdx.x;
+  if ((nt & 7) == 0) bid = (bid & 7) * (nt >> 3) + (bid >> 3);
+  int per_group = R_GROUP_M * tiles_n;
+  int group = bid / per_group;
+  int first_m = group * R_GROUP_M;
+  int gsz = min(tiles_m - first_m, R_GROUP_M);
+  int in_group = bid - group * per_group;
+  tm = first_m + in_group % gsz;
+  tn = in_group / gsz;
+}
+
+// EDGE = false requires M % 128 == 0, N % 128 == 0, K % 16 == 0 and 16-byte
+// aligned A, B (the host checks); EDGE = true guards every element.
+template <typename T, bool CPLX, bool EDGE>
+__global__ __launch_bounds__(R_THREADS) void gemm_real_kernel(RealArgs<T> p) {
+  using MF = Mfma16x16x4<T>;
+  using acc_t = typename MF::acc_t;
+  __shared__ T As[2][RB_K][RB_M];
+  __shared__ T Bs[2][RB_K][RB_N];
+
+  const int M = p.M, N = p.N, K = p.K;
+  const int tiles_m = (M + RB_M - 1) / RB_M, tiles_n = (N + RB_N - 1) / RB_N;
+  int tm, tn;
+  tile_coords(tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * RB_M, n0 = tn * RB_N;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  // A loader: row tid/2, 8 consecutive k at (tid&1)*8.
+  const int a_row = tid >> 1, a_k = (tid & 1) * 8;
+  // B loader (real): k row tid/16, 8 consecutive n at (tid&15)*8.
+  const int b_k = tid >> 4, b_n = (tid & 15) * 8;
+  // B loader (complex): complex row tid/32, 2 complex cols at (tid&31)*2.
+  const int cb_k = tid >> 5, cb_n = (tid & 31) * 2;
+
+  T ra[8], rb[8];
+
+  auto load_a = [&](int k0) {
+    const int gr = m0 + a_row;
+    if (!EDGE) {
+      load8(p.a + (size_t)gr * K + k0 + a_k, ra);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        int gk = k0 + a_k + q;
+        ra[q] = (gr < M && gk < K) ? p.a[(size_t)gr * K + gk] : T(0);
+      }
+    }
+  };
+  auto load_b = [&](int k0) {
+    if (!CPLX) {
+      const int gk = k0 + b_k;
+      if (!EDGE) {
+        load8(p.b + (size_t)gk * N + n0 + b_n, rb);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          int gn = n0 + b_n + q;
+          rb[q] = (gk < K && gn < N) ? p.b[(size_t)gk * N + gn] : T(0);
+        }
+      }
+    } else {
+      // complex row (k0/2 + cb_k), complex cols n0/2 + cb_n, +1
+      const int kc = (k0 >> 1) + cb_k, Kc = K >> 1, Nc = N >> 1;
+      const int nc = (n0 >> 1) + cb_n;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        bool ok = !EDGE || (kc < Kc && nc + q < Nc);
+        const T* src = p.b + ((size_t)kc * Nc + nc + q) * 2;
+        rb[2 * q] = ok ? src[0] : T(0);
+        rb[2 * q + 1] = ok ? src[1] : T(0);
+      }
+    }
+  };
+  auto store_tiles = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) As[buf][a_k + q][a_row] = ra[q];
+    if (!CPLX) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) Bs[buf][b_k][b_n + q] = rb[q];
+    } else {
+      const int r0 = 2 * cb_k, c0 = 2 * cb_n;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        T br = rb[2 * q], bi = rb[2 * q + 1];
+        Bs[buf][r0][c0 + 2 * q] = br;
+        Bs[buf][r0][c0 + 2 * q + 1] = bi;
+        Bs[buf][r0 + 1][c0 + 2 * q] = -bi;
+        Bs[buf][r0 + 1][c0 + 2 * q + 1] = br;
+      }
+    }
+  };
+
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  const int nk = (K + RB_K - 1) / RB_K;
+  load_a(0);
+  load_b(0);
+  store_tiles(0);
+  __syncthreads();
+
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) {
+      load_a((kt + 1) * RB_K);
+      load_b((kt + 1) * RB_K);
+    }
+#pragma unroll
+    for (int kk = 0; kk < RB_K / 4; ++kk) {
+      T af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = As[cur][kk * 4 + lk][wm0 + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = Bs[cur][kk * 4 + lk][wn0 + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = MF::run(af[i], bf[j], acc[i][j]);
+    }
+    if (more) store_tiles(cur ^ 1);
+    __syncthreads();
+  }
+
+  // Epilogue.  acc[i][j][r] holds C[m0+wm0+16i+4*lk+r][n0+wn0+16j+lr].
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = n0 + wn0 + j * 16 + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm0 + i * 16 + lk * 4 + r;
+        T v = acc[i][j][r];
+        if (!CPLX) {
+          if (EDGE && (row >= M || col >= N)) continue;
+          T* dst = p.c + (size_t)row * N + col;
+          if (!p.alpha_one) v *= p.alpha_re;
+          if (!p.beta_zero) v = fma(p.beta_re, *dst, v);
+          *dst = v;
+        } else {
+          // lanes (2t, 2t+1) hold (re, im) of one complex element
+          T other = __shfl_xor(v, 1);
+          const bool odd = lr & 1;
+          T re = odd ? other : v, im = odd ? v : other;
+          if (EDGE && (row >= M || col >= N)) continue;
+          T outv;
+          if (!p.alpha_one) {
+            T vr = p.alpha_re * re - p.alpha_im * im;
+            T vi = p.alpha_re * im + p.alpha_im * re;
+            re = vr;
+            im = vi;
+          }
+          T* dst = p.c + (size_t)row * N + (col & ~1);
+          if (!p.beta_zero) {
+            T cr = dst[0], ci = dst[1];
+            T sr = p.beta_re * cr - p.beta_im * ci;
+            T si = p.beta_re * ci + p.beta_im * cr;
+            outv = odd ? (im + si) : (re + sr);
+          } else {
+            outv = odd ? im : re;
+          }
+          dst[odd ? 1 : 0] = outv;
+        }
+      }
+    }
+  }
+}
+
+// C = beta * C for k == 0 (outer.rs:214-231, :1249-1266).
+template <typename T, bool CPLX>
+__global__ void scale_c_kernel(T* c, size_t count, T beta_re, T beta_im) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) {
+    if (!CPLX) {
+      c[i] = (beta_re == T(0)) ? T(0) : beta_re * c[i];
+    } else {
+      T* e = c + 2 * i;
+      if (beta_re == T(0) && beta_im == T(0)) {
+        e[0] = T(0);
+        e[1] = T(0);
+      } else {
+        T cr = e[0], ci = e[1];
+        e[0] = beta_re * cr - beta_im * ci;
+        e[1] = beta_re * ci + beta_im * cr;
+      }
+    }
+  }
+}
+
+template <typename T, bool CPLX>
+static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, const T* b,
+                       T br, T bi, T* c, hipStream_t st) {
+  if (m == 0 || n == 0) return FERRAY_OK;
+  if (!c || (k && (!a || !b))) return FERRAY_ERR_INVALID;
+  const size_t mult = CPLX ? 2 : 1;
+  if (m > (1u << 30) || n * mult > (1u << 30) || k * mult > (1u << 30)) return FERRAY_ERR_INVALID;
+  if (k == 0) {
+    const bool beta_one = br == T(1) && bi == T(0);
+    if (beta_one) return FERRAY_OK;
+    size_t count = m * n;
+    size_t blocks = cdiv(count, 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL((scale_c_kernel<T, CPLX>), dim3((unsigned)blocks), dim3(256), 0, st, c,
+                       count, br, bi);
+    FG_CHECK(hipGetLastError());
+    return FERRAY_OK;
+  }
+  RealArgs<T> p;
+  p.a = a;
+  p.b = b;
+  p.c = c;
+  p.M = (int)m;
+  p.N = (int)(n * mult);
+  p.K = (int)(k * mult);
+  p.alpha_re = ar;
+  p.alpha_im = ai;
+  p.beta_re = br;
+  p.beta_im = bi;
+  p.alpha_one = (ar == T(1) && ai == T(0));
+  p.beta_zero = (br == T(0) && bi == T(0));
+  const size_t tiles = cdiv(m, RB_M) * cdiv(n * mult, RB_N);
+  if (tiles > 0x7fffffff) return FERRAY_ERR_INVALID;
+  const bool aligned = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0);
+  const bool fast = aligned && m % RB_M == 0 && (n * mult) % RB_N == 0 && (k * mult) % RB_K == 0;
+  if (fast)
+    hipLaunchKernelGGL((gemm_real_kernel<T, CPLX, false>), dim3((unsigned)tiles), dim3(R_THREADS),
+                       0, st, p);
+  else
+    hipLaunchKernelGGL((gemm_real_kernel<T, CPLX, true>), dim3((unsigned)tiles), dim3(R_THREADS),
+                       0, st, p);
+  FG_CHECK(hipGetLastError());
+  return FERRAY_OK;
+}
+
+}  // namespace fg
+
+using namespace fg;
+
+extern "C" {
+
+int ferray_gemm_f64_dev(size_t m, size_t n, size_t k, double alpha, const double* a,
+                        const double* b, double beta, double* c, void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  return launch_real<double, false>(m, n, k, alpha, 0.0, a, b, beta, 0.0, c,
+                                    resolve_stream(stream));
+}
+
+int ferray_gemm_f32_dev(size_t m, size_t n, size_t k, float alpha, const float* a,
+                        const float* b, float beta, float* c, void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  return launch_real<float, false>(m, n, k, alpha, 0.0f, a, b, beta, 0.0f, c,
+                                   resolve_stream(stream));
+}
+
+int ferray_gemm_c64_dev(size_t m, size_t n, size_t k, double alpha_re, double alpha_im,
+                        const double* a, const double* b, double beta_re, double beta_im,
+                        double* c, void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  return launch_real<double, true>(m, n, k, alpha_re, alpha_im, a, b, beta_re, beta_im, c,
+                                   resolve_stream(stream));
+}
+
+int ferray_gemm_c32_dev(size_t m, size_t n, size_t k, float alpha_re, float alpha_im,
+                        const float* a, const float* b, float beta_re, float beta_im, float* c,
+                        void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  return launch_real<float, true>(m, n, k, alpha_re, alpha_im, a, b, beta_re, beta_im, c,
+                                  resolve_stream(stream));
+}
+
+}  // extern "C"

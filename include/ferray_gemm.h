@@ -1,0 +1,155 @@
+/*
+ * ferray_gemm.h — C ABI of the MI355X (gfx950) GEMM library.
+ *
+ * The first block mirrors ferray-linalg's BLAS-style GEMM entry points one for
+ * one (ferray-linalg/src/gemm/mod.rs): same parameter order and meaning,
+ * tightly packed row-major operands (lda = k, ldb = ldc = n), interleaved
+ * [re, im] complex storage, host pointers borrowed for the duration of a
+ * synchronous call, and a `bool` result that is false iff the backend could
+ * not run (no device, allocation failure, launch failure), in which case the
+ * caller falls back to its CPU path exactly as ferray's callers do when a
+ * hand-tuned kernel returns false (products/mod.rs:509-520, complex.rs:189).
+ * Buffers must not overlap (mod.rs:342-346).  All calls are thread safe; each
+ * calling thread gets its own HIP stream.
+ *
+ * The `_dev` block takes device pointers and a hipStream_t (passed as void*,
+ * NULL = the calling thread's library stream), enqueues work and returns a
+ * status code without synchronising.
+ */
+#ifndef FERRAY_GEMM_H
+#define FERRAY_GEMM_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes of the `_dev` entry points and ferray_gemm_last_status(). */
+#define FERRAY_OK 0
+#define FERRAY_ERR_NO_DEVICE 1
+#define FERRAY_ERR_OOM 2
+#define FERRAY_ERR_INVALID 3
+#define FERRAY_ERR_HIP 4
+#define FERRAY_ERR_UNSUPPORTED 5
+
+/* ---------------- drop-in entry points (host pointers) ---------------- */
+
+/* C = alpha*A@B + beta*C, f64.  Replaces gemm::gemm_f64 (mod.rs:197-328). */
+bool ferray_gemm_f64(size_t m, size_t n, size_t k, double alpha,
+                     const double *a, const double *b, double beta, double *c);
+/* f32 sibling.  Replaces gemm::gemm_f32 (mod.rs:940-1062). */
+bool ferray_gemm_f32(size_t m, size_t n, size_t k, float alpha, const float *a,
+                     const float *b, float beta, float *c);
+/* Complex<f64>, interleaved.  Replaces gemm::gemm_c64 (mod.rs:757-826). */
+bool ferray_gemm_c64(size_t m, size_t n, size_t k, double alpha_re,
+                     double alpha_im, const double *a, const double *b,
+                     double beta_re, double beta_im, double *c);
+/* Complex<f32>, interleaved.  Replaces gemm::gemm_c32 (mod.rs:669-736). */
+bool ferray_gemm_c32(size_t m, size_t n, size_t k, float alpha_re,
+                     float alpha_im, const float *a, const float *b,
+                     float beta_re, float beta_im, float *c);
+/* u8 x i8 -> i32, exact wrapping (the VNNI / NEON semantics).
+ * Replaces gemm::gemm_i8 (mod.rs:601-651). */
+bool ferray_gemm_i8(size_t m, size_t n, size_t k, const uint8_t *a,
+                    const int8_t *b, int32_t *c, bool accumulate);
+/* i8 x i8 -> i32 wrapping.  Replaces gemm::gemm_i8_signed (mod.rs:414-448). */
+bool ferray_gemm_i8_signed(size_t m, size_t n, size_t k, const int8_t *a,
+                           const int8_t *b, int32_t *c, bool accumulate);
+/* i16 x i16 -> i32 wrapping (mod 2^32).  Replaces gemm::gemm_i16
+ * (mod.rs:348-392). */
+bool ferray_gemm_i16(size_t m, size_t n, size_t k, const int16_t *a,
+                     const int16_t *b, int32_t *c, bool accumulate);
+/* Zero-point / scale quantized matmul, f32 out.  Replaces
+ * gemm::quantized_matmul (mod.rs:484-575). */
+bool ferray_quantized_matmul(size_t m, size_t n, size_t k, const uint8_t *a,
+                             uint8_t a_zp, float scale_a, const int8_t *b,
+                             int8_t b_zp, float scale_b, float *c,
+                             bool accumulate);
+/* bf16 x bf16 -> f32 (raw bf16 bits).  Replaces gemm::gemm_bf16_f32
+ * (mod.rs:851-882). */
+bool ferray_gemm_bf16_f32(size_t m, size_t n, size_t k, const uint16_t *a,
+                          const uint16_t *b, float *c, bool accumulate);
+/* f16 x f16 -> f32 (raw binary16 bits).  Replaces gemm::gemm_f16_f32
+ * (mod.rs:906-933). */
+bool ferray_gemm_f16_f32(size_t m, size_t n, size_t k, const uint16_t *a,
+                         const uint16_t *b, float *c, bool accumulate);
+
+/* Device probe: the analogue of gemm::cpu_supports_avx2_fma
+ * (mod.rs:118-188) — true iff a gfx950 device is usable. */
+bool ferray_gpu_available(void);
+/* Status of the calling thread's last drop-in call (FERRAY_OK ...). */
+int ferray_gemm_last_status(void);
+/* Human-readable text for a status code. */
+const char *ferray_status_string(int status);
+
+/* ---------------- device-pointer entry points ---------------- */
+
+int ferray_gemm_f64_dev(size_t m, size_t n, size_t k, double alpha,
+                        const double *a, const double *b, double beta,
+                        double *c, void *stream);
+int ferray_gemm_f32_dev(size_t m, size_t n, size_t k, float alpha,
+                        const float *a, const float *b, float beta, float *c,
+                        void *stream);
+int ferray_gemm_c64_dev(size_t m, size_t n, size_t k, double alpha_re,
+                        double alpha_im, const double *a, const double *b,
+                        double beta_re, double beta_im, double *c,
+                        void *stream);
+int ferray_gemm_c32_dev(size_t m, size_t n, size_t k, float alpha_re,
+                        float alpha_im, const float *a, const float *b,
+                        float beta_re, float beta_im, float *c, void *stream);
+int ferray_gemm_i8_dev(size_t m, size_t n, size_t k, const uint8_t *a,
+                       const int8_t *b, int32_t *c, bool accumulate,
+                       void *stream);
+int ferray_gemm_i8_signed_dev(size_t m, size_t n, size_t k, const int8_t *a,
+                              const int8_t *b, int32_t *c, bool accumulate,
+                              void *stream);
+int ferray_gemm_i16_dev(size_t m, size_t n, size_t k, const int16_t *a,
+                        const int16_t *b, int32_t *c, bool accumulate,
+                        void *stream);
+int ferray_quantized_matmul_dev(size_t m, size_t n, size_t k,
+                                const uint8_t *a, uint8_t a_zp, float scale_a,
+                                const int8_t *b, int8_t b_zp, float scale_b,
+                                float *c, bool accumulate, void *stream);
+int ferray_gemm_bf16_f32_dev(size_t m, size_t n, size_t k, const uint16_t *a,
+                             const uint16_t *b, float *c, bool accumulate,
+                             void *stream);
+int ferray_gemm_f16_f32_dev(size_t m, size_t n, size_t k, const uint16_t *a,
+                            const uint16_t *b, float *c, bool accumulate,
+                            void *stream);
+
+/* ---------------- device memory, streams, timing ---------------- */
+
+int ferray_dev_malloc(size_t bytes, void **out);
+int ferray_dev_free(void *ptr);
+int ferray_dev_upload(void *dst, const void *src, size_t bytes, void *stream);
+int ferray_dev_download(void *dst, const void *src, size_t bytes, void *stream);
+int ferray_dev_memset(void *dst, int value, size_t bytes, void *stream);
+int ferray_stream_sync(void *stream);
+/* The calling thread's library stream (what NULL resolves to). */
+void *ferray_default_stream(void);
+
+/* HIP events for timing work on a stream (elapsed time in milliseconds). */
+int ferray_event_create(void **out);
+int ferray_event_record(void *event, void *stream);
+int ferray_event_elapsed_ms(void *start, void *stop, float *ms);
+int ferray_event_destroy(void *event);
+int ferray_device_sync(void);
+
+/* Seeded device generators (SplitMix64 counter stream, element i of stream
+ * `seed` = mix(seed + (i+1)*0x9E3779B97F4A7C15)).
+ *   kind 0: f64 uniform [-1,1)          kind 1: f64 normal(0,1) (Box-Muller)
+ *   kind 2: f64 u*2^e, u uniform[-1,1), e uniform integer in [lo, hi]
+ *   kind 3: f32 uniform [-1,1)          kind 4: f32 normal(0,1)
+ *   kind 5: i8  uniform integer [lo, hi]   kind 6: i16 uniform integer [lo, hi]
+ *   kind 7: u8  uniform integer [lo, hi]
+ * `count` counts scalar elements (complex: 2 per element). */
+int ferray_dev_fill(void *dst, size_t count, int kind, uint64_t seed,
+                    int64_t lo, int64_t hi, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FERRAY_GEMM_H */
