@@ -25,12 +25,16 @@ namespace fg {
 
 template <typename T>
 struct Mfma16x16x4;
+// Operand layout of both: lane l supplies A[l%16][l/16] and B[l/16][l%16].
+// Result element r of lane l is C[row(l/16, r)][l%16]; the two instructions
+// lay the rows out differently (measured on gfx950).
 template <>
 struct Mfma16x16x4<double> {
   using acc_t = d4;
   static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
 };
 template <>
 struct Mfma16x16x4<float> {
@@ -38,6 +42,7 @@ struct Mfma16x16x4<float> {
   static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
 // Eight consecutive elements from a 16-byte aligned address as 128-bit loads.
@@ -200,7 +205,7 @@ __global__ __launch_bounds__(R_THREADS) void gemm_real_kernel(RealArgs<T> p) {
     __syncthreads();
   }
 
-  // Epilogue.  acc[i][j][r] holds C[m0+wm0+16i+4*lk+r][n0+wn0+16j+lr].
+  // Epilogue.  acc[i][j][r] holds C[m0+wm0+16i+row(lk,r)][n0+wn0+16j+lr].
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -208,7 +213,7 @@ __global__ __launch_bounds__(R_THREADS) void gemm_real_kernel(RealArgs<T> p) {
       const int col = n0 + wn0 + j * 16 + lr;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm0 + i * 16 + lk * 4 + r;
+        const int row = m0 + wm0 + i * 16 + MF::row(lk, r);
         T v = acc[i][j][r];
         if (!CPLX) {
           if (EDGE && (row >= M || col >= N)) continue;
