@@ -12,7 +12,8 @@
 // tile is written to LDS.  C_r is C's interleaved memory, so there is no extra
 // HBM pass and the real flop count is exactly 8mnk.
 //
-// Block: 256 threads (4 waves, 2x2), block tile 128x128, BK = 16, each wave
+// Block: 256 threads (4 waves, 2x2), block tile 128x128, BK = 8 (f64) or 16
+// (f32), each wave
 // owns a 64x64 sub-tile = 4x4 MFMA tiles of 16x16.  A and B panels go through
 // registers into double-buffered LDS (one barrier per k-step).  Tiles are
 // ordered in groups of GROUP_M rows of tiles, and the linear block id is
@@ -45,20 +46,6 @@ struct Mfma16x16x4<float> {
   static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
-// Eight consecutive elements from a 16-byte aligned address as 128-bit loads.
-template <typename T>
-__device__ __forceinline__ void load8(const T* src, T (&dst)[8]) {
-  constexpr int V = 16 / sizeof(T);
-  typedef T vec_t __attribute__((ext_vector_type(V)));
-  const vec_t* s = reinterpret_cast<const vec_t*>(src);
-#pragma unroll
-  for (int q = 0; q < 8 / V; ++q) {
-    vec_t v = s[q];
-#pragma unroll
-    for (int e = 0; e < V; ++e) dst[q * V + e] = v[e];
-  }
-}
-
 template <typename T>
 struct RealArgs {
   const T* a;
@@ -70,7 +57,13 @@ struct RealArgs {
   int beta_zero;  // beta == 0: C is written without being read
 };
 
-constexpr int RB_M = 128, RB_N = 128, RB_K = 16, R_THREADS = 256, R_GROUP_M = 8;
+constexpr int RB_M = 128, RB_N = 128, R_THREADS = 256, R_GROUP_M = 8;
+// k-depth of one pipeline stage: 8 for f64 (32 KiB of LDS, so two blocks fit
+// on a CU), 16 for f32.
+template <typename T>
+struct StageK {
+  static constexpr int value = sizeof(T) == 8 ? 8 : 16;
+};
 
 __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
   int nt = tiles_m * tiles_n;
@@ -85,14 +78,22 @@ __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int& tm, i
   tn = in_group / gsz;
 }
 
-// EDGE = false requires M % 128 == 0, N % 128 == 0, K % 16 == 0 and 16-byte
+// EDGE = false requires M % 128 == 0, N % 128 == 0, K % BK == 0 and 16-byte
 // aligned A, B (the host checks); EDGE = true guards every element.
 template <typename T, bool CPLX, bool EDGE>
-__global__ __launch_bounds__(R_THREADS) void gemm_real_kernel(RealArgs<T> p) {
+__global__ __launch_bounds__(R_THREADS) __attribute__((amdgpu_waves_per_eu(2, 8)))
+void gemm_real_kernel(RealArgs<T> p) {
   using MF = Mfma16x16x4<T>;
   using acc_t = typename MF::acc_t;
-  __shared__ T As[2][RB_K][RB_M];
-  __shared__ T Bs[2][RB_K][RB_N];
+  constexpr int BK = StageK<T>::value;
+  constexpr int V = 16 / sizeof(T);                  // elements per 16-byte vector
+  constexpr int AV = RB_M * BK / V / R_THREADS;      // A vectors per thread
+  constexpr int BV = BK * RB_N / V / R_THREADS;      // B vectors per thread (real)
+  constexpr int CV = BK * RB_N / (2 * V) / R_THREADS;  // B vectors per thread (complex)
+  static_assert(AV >= 1 && BV >= 1 && CV >= 1, "loader shape");
+  typedef T vec_t __attribute__((ext_vector_type(V)));
+  __shared__ T As[2][BK][RB_M];
+  __shared__ T Bs[2][BK][RB_N];
 
   const int M = p.M, N = p.N, K = p.K;
   const int tiles_m = (M + RB_M - 1) / RB_M, tiles_n = (N + RB_N - 1) / RB_N;
@@ -105,67 +106,89 @@ __global__ __launch_bounds__(R_THREADS) void gemm_real_kernel(RealArgs<T> p) {
   const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
   const int lr = lane & 15, lk = lane >> 4;
 
-  // A loader: row tid/2, 8 consecutive k at (tid&1)*8.
-  const int a_row = tid >> 1, a_k = (tid & 1) * 8;
-  // B loader (real): k row tid/16, 8 consecutive n at (tid&15)*8.
-  const int b_k = tid >> 4, b_n = (tid & 15) * 8;
-  // B loader (complex): complex row tid/32, 2 complex cols at (tid&31)*2.
-  const int cb_k = tid >> 5, cb_n = (tid & 31) * 2;
-
-  T ra[8], rb[8];
+  // Vector v of the A tile covers row v/(BK/V), k = (v%(BK/V))*V .. +V; vector
+  // v of the B tile covers k row v/(BN/V), n = (v%(BN/V))*V .. +V (complex:
+  // complex row v/(BN/V), complex cols (v%(BN/V))*V/2 .. +V/2).
+  vec_t ra[AV], rb[CPLX ? CV : BV];
 
   auto load_a = [&](int k0) {
-    const int gr = m0 + a_row;
-    if (!EDGE) {
-      load8(p.a + (size_t)gr * K + k0 + a_k, ra);
-    } else {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        int gk = k0 + a_k + q;
-        ra[q] = (gr < M && gk < K) ? p.a[(size_t)gr * K + gk] : T(0);
+    for (int q = 0; q < AV; ++q) {
+      const int v = tid + q * R_THREADS, row = v / (BK / V), kc = (v % (BK / V)) * V;
+      const int gr = m0 + row;
+      if (!EDGE) {
+        ra[q] = *reinterpret_cast<const vec_t*>(p.a + (size_t)gr * K + k0 + kc);
+      } else {
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const int gk = k0 + kc + e;
+          ra[q][e] = (gr < M && gk < K) ? p.a[(size_t)gr * K + gk] : T(0);
+        }
       }
     }
   };
   auto load_b = [&](int k0) {
     if (!CPLX) {
-      const int gk = k0 + b_k;
-      if (!EDGE) {
-        load8(p.b + (size_t)gk * N + n0 + b_n, rb);
-      } else {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          int gn = n0 + b_n + q;
-          rb[q] = (gk < K && gn < N) ? p.b[(size_t)gk * N + gn] : T(0);
+      for (int q = 0; q < BV; ++q) {
+        const int v = tid + q * R_THREADS, kr = v / (RB_N / V), nc = (v % (RB_N / V)) * V;
+        const int gk = k0 + kr;
+        if (!EDGE) {
+          rb[q] = *reinterpret_cast<const vec_t*>(p.b + (size_t)gk * N + n0 + nc);
+        } else {
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            const int gn = n0 + nc + e;
+            rb[q][e] = (gk < K && gn < N) ? p.b[(size_t)gk * N + gn] : T(0);
+          }
         }
       }
     } else {
-      // complex row (k0/2 + cb_k), complex cols n0/2 + cb_n, +1
-      const int kc = (k0 >> 1) + cb_k, Kc = K >> 1, Nc = N >> 1;
-      const int nc = (n0 >> 1) + cb_n;
+      const int Kc = K >> 1, Nc = N >> 1;
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        bool ok = !EDGE || (kc < Kc && nc + q < Nc);
-        const T* src = p.b + ((size_t)kc * Nc + nc + q) * 2;
-        rb[2 * q] = ok ? src[0] : T(0);
-        rb[2 * q + 1] = ok ? src[1] : T(0);
+      for (int q = 0; q < CV; ++q) {
+        const int v = tid + q * R_THREADS, kr = v / (RB_N / V), cc = (v % (RB_N / V)) * (V / 2);
+        const int kc = (k0 >> 1) + kr, nc = (n0 >> 1) + cc;
+        if (!EDGE) {
+          rb[q] = *reinterpret_cast<const vec_t*>(p.b + ((size_t)kc * Nc + nc) * 2);
+        } else {
+#pragma unroll
+          for (int e = 0; e < V / 2; ++e) {
+            const bool ok = kc < Kc && nc + e < Nc;
+            const T* src = p.b + ((size_t)kc * Nc + nc + e) * 2;
+            rb[q][2 * e] = ok ? src[0] : T(0);
+            rb[q][2 * e + 1] = ok ? src[1] : T(0);
+          }
+        }
       }
     }
   };
   auto store_tiles = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) As[buf][a_k + q][a_row] = ra[q];
+    for (int q = 0; q < AV; ++q) {
+      const int v = tid + q * R_THREADS, row = v / (BK / V), kc = (v % (BK / V)) * V;
+#pragma unroll
+      for (int e = 0; e < V; ++e) As[buf][kc + e][row] = ra[q][e];
+    }
     if (!CPLX) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) Bs[buf][b_k][b_n + q] = rb[q];
+      for (int q = 0; q < BV; ++q) {
+        const int v = tid + q * R_THREADS, kr = v / (RB_N / V), nc = (v % (RB_N / V)) * V;
+        *reinterpret_cast<vec_t*>(&Bs[buf][kr][nc]) = rb[q];
+      }
     } else {
-      const int r0 = 2 * cb_k, c0 = 2 * cb_n;
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        T br = rb[2 * q], bi = rb[2 * q + 1];
-        Bs[buf][r0][c0 + 2 * q] = br;
-        Bs[buf][r0][c0 + 2 * q + 1] = bi;
-        Bs[buf][r0 + 1][c0 + 2 * q] = -bi;
-        Bs[buf][r0 + 1][c0 + 2 * q + 1] = br;
+      for (int q = 0; q < CV; ++q) {
+        const int v = tid + q * R_THREADS, kr = v / (RB_N / V), cc = (v % (RB_N / V)) * (V / 2);
+        const int r0 = 2 * kr, c0 = 2 * cc;
+#pragma unroll
+        for (int e = 0; e < V / 2; ++e) {
+          const T br = rb[q][2 * e], bi = rb[q][2 * e + 1];
+          Bs[buf][r0][c0 + 2 * e] = br;
+          Bs[buf][r0][c0 + 2 * e + 1] = bi;
+          Bs[buf][r0 + 1][c0 + 2 * e] = -bi;
+          Bs[buf][r0 + 1][c0 + 2 * e + 1] = br;
+        }
       }
     }
   };
@@ -176,7 +199,7 @@ __global__ __launch_bounds__(R_THREADS) void gemm_real_kernel(RealArgs<T> p) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  const int nk = (K + RB_K - 1) / RB_K;
+  const int nk = (K + BK - 1) / BK;
   load_a(0);
   load_b(0);
   store_tiles(0);
@@ -186,11 +209,11 @@ __global__ __launch_bounds__(R_THREADS) void gemm_real_kernel(RealArgs<T> p) {
     const int cur = kt & 1;
     const bool more = kt + 1 < nk;
     if (more) {
-      load_a((kt + 1) * RB_K);
-      load_b((kt + 1) * RB_K);
+      load_a((kt + 1) * BK);
+      load_b((kt + 1) * BK);
     }
 #pragma unroll
-    for (int kk = 0; kk < RB_K / 4; ++kk) {
+    for (int kk = 0; kk < BK / 4; ++kk) {
       T af[4], bf[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) af[i] = As[cur][kk * 4 + lk][wm0 + i * 16 + lr];
@@ -306,7 +329,8 @@ static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, con
   const size_t tiles = cdiv(m, RB_M) * cdiv(n * mult, RB_N);
   if (tiles > 0x7fffffff) return FERRAY_ERR_INVALID;
   const bool aligned = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0);
-  const bool fast = aligned && m % RB_M == 0 && (n * mult) % RB_N == 0 && (k * mult) % RB_K == 0;
+  const bool fast = aligned && m % RB_M == 0 && (n * mult) % RB_N == 0 &&
+                    (k * mult) % StageK<T>::value == 0;
   if (fast)
     hipLaunchKernelGGL((gemm_real_kernel<T, CPLX, false>), dim3((unsigned)tiles), dim3(R_THREADS),
                        0, st, p);

@@ -131,8 +131,10 @@ def test_dgemm_config1_lcg_512(gpu):
     oracle.gemm_f64(n, n, n, 1.0, a, b, 0.0, ref)
     bound = _bound(a, b, n, n, n, 1.0, 0.0, np.zeros(n * n), U64)
     assert np.all(np.abs(c - ref).reshape(n, n) <= bound)
+    # make_matrix yields values in [-1, 0), so ||C|| ~ 0.75 ||A|| ||B|| here and
+    # both norms measure the same few-ulp relative error.
     e_ab, e_c = normwise_error(c, ref, a, b)
-    assert e_ab < 1e-16 and e_c < 1e-14, (e_ab, e_c)
+    assert e_ab < 4e-15 and e_c < 4e-15, (e_ab, e_c)
 
 
 # ---------------------------------------------------------------- complex
