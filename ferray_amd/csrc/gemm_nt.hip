@@ -1,17 +1,16 @@
-// Integer and half-precision GEMM on CDNA4 matrix cores.
+// Integer and half-precision GEMM entry points on the NT matrix-core main
+// loop (mfma_nt.hpp).
 //
-//   i8 x i8 -> i32    v_mfma_i32_16x16x64_i8    (gemm_i8_signed)
-//   u8 x i8 -> i32    same, A shifted by -128 plus 128*colsum(B)  (gemm_i8)
-//   i16 x i16 -> i32  four i8 products of byte planes              (gemm_i16)
-//   quantized_matmul  u8 x i8 core with a fused f32 epilogue
-//   bf16/f16 -> f32   v_mfma_f32_16x16x32_{bf16,f16}
+//   i8 x i8 -> i32    one i8 pass                              (gemm_i8_signed)
+//   u8 x i8 -> i32    A's sign bit flipped on load (a - 128), plus 128*colsum(B)
+//                                                                      (gemm_i8)
+//   i16 x i16 -> i32  four i8 passes over byte planes                 (gemm_i16)
+//   quantized_matmul  the u8 x i8 pass with a fused zero-point / scale epilogue
+//   bf16/f16 -> f32   one half-precision pass                 (gemm_{bf16,f16}_f32)
 //
-// All of these MFMAs take 16 bytes of K per lane, so one "NT" kernel serves
-// them: both operands are k-contiguous (A is m x k row-major, B is used as its
-// transpose Bt, n x k).  B is transposed (and split / summed where the type
-// needs it) by a prepass into library workspace; A is read in place when its
-// shape allows, otherwise packed the same way.  Padding is zero, so the main
-// kernel never bounds-checks a load.
+// B is transposed into k-contiguous planes by a prepass (which also forms the
+// i16 byte planes and the wrapping column sums); A is read in place when its
+// shape allows, otherwise packed the same way.  Padding is zero.
 //
 // Integer arithmetic is exact mod 2^32:
 //   u8:  sum a*b = sum (a-128)*b + 128*sum_k b[k][j]
@@ -19,275 +18,152 @@
 //        sum a*b = 65536*S(ah,bh) + 256*(S(ah,bl') + S(al',bh)) + S(al',bl')
 //                  + 128*rowsum(A) + 128*colsum(B) - 16384*k
 // where every S(.,.) is an i8 x i8 MFMA product; all terms wrap in uint32.
-#include "common.hpp"
+// K is reduced in chunks of at most 2^16 so that no i32 accumulator can
+// leave the i32 range whatever the operands (|a*b| <= 2^14 per term).
+#include <algorithm>
+
+#include "mfma_nt.hpp"
 
 namespace fg {
 
 enum NtKind { NT_I8S = 0, NT_U8 = 1, NT_I16 = 2, NT_BF16 = 3, NT_F16 = 4, NT_QUANT = 5 };
 
-constexpr int NT_THREADS = 256;
-constexpr int NT_KB = 64;         // bytes of k per pipeline stage
-constexpr int NT_ROW = NT_KB + 16;  // LDS row pitch in bytes (16 B pad)
+constexpr size_t NT_K_CHUNK = 1u << 16;
 
-struct NtArgs {
-  const uint8_t* a0;  // plane 0 of A (m rows, lda bytes each)
-  const uint8_t* a1;  // plane 1 of A (i16 low bytes) or null
-  const uint8_t* b0;  // plane 0 of Bt (n rows, ldb bytes each)
-  const uint8_t* b1;
-  size_t lda, ldb;    // row pitch in bytes
-  int kbytes;         // bytes of k to reduce (multiple of NT_KB, zero padded)
-  void* c;            // i32 or f32 output, m x n row-major
-  int M, N, K;        // real sizes (K in elements)
-  int accumulate;
-  int xor_a;          // flip the sign bit of every A byte on load (u8 -> u8-128)
-  const int32_t* rowsum;  // i16: rowsum(A); quant: rowsum(A) of u8
-  const int32_t* colsum;  // u8/quant/i16: colsum(B)
-  int32_t azp, bzp;
-  float scale;
+// ---------------------------------------------------------------------
+// Epilogues
+// ---------------------------------------------------------------------
+
+// i32 output.  v = (acc << shift) + 128*colsum[col] (if colsum) + corr[row]
+// (if rowcorr) + kconst + (old C if read_c).
+struct EpiI32 {
+  int32_t* c;
+  int M, N;
+  int shift;
+  int read_c;
+  const int32_t* colsum;   // u8: 128*colsum(B); i16: 128*colsum(B)
+  const int32_t* rowsum;   // i16: 128*rowsum(A)
+  uint32_t kconst;         // i16: -16384*k
+  __device__ __forceinline__ void operator()(int row, int col, int, int acc) const {
+    if (row >= M || col >= N) return;
+    uint32_t v = (uint32_t)acc << shift;
+    if (colsum) v += 128u * (uint32_t)colsum[col];
+    if (rowsum) v += 128u * (uint32_t)rowsum[row];
+    v += kconst;
+    int32_t* dst = c + (size_t)row * N + col;
+    if (read_c) v += (uint32_t)*dst;
+    *dst = (int32_t)v;
+  }
 };
 
-__device__ __forceinline__ void nt_tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
-  const int G = 8;
-  int nt = tiles_m * tiles_n;
-  int bid = blockIdx.x;
-  if ((nt & 7) == 0) bid = (bid & 7) * (nt >> 3) + (bid >> 3);
-  int per_group = G * tiles_n;
-  int group = bid / per_group;
-  int first_m = group * G;
-  int gsz = min(tiles_m - first_m, G);
-  int in_group = bid - group * per_group;
-  tm = first_m + in_group % gsz;
-  tn = in_group / gsz;
-}
-
-template <int KIND>
-__device__ __forceinline__ f4 mfma_half(uint4 a, uint4 b, f4 c) {
-  if constexpr (KIND == NT_BF16)
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a),
-                                                   __builtin_bit_cast(bf8, b), c, 0, 0, 0);
-  else
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a),
-                                                  __builtin_bit_cast(h8, b), c, 0, 0, 0);
-}
-__device__ __forceinline__ i4 mfma_i8(uint4 a, uint4 b, i4 c) {
-  return __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(i4, a),
-                                               __builtin_bit_cast(i4, b), c, 0, 0, 0);
-}
-
-// BM x BN block tile, 4 waves in 2x2, each wave (BM/2) x (BN/2).
-template <int KIND, int BM, int BN>
-__global__ __launch_bounds__(NT_THREADS) void gemm_nt_kernel(NtArgs p) {
-  constexpr bool HALF = (KIND == NT_BF16 || KIND == NT_F16);
-  constexpr int PL = (KIND == NT_I16) ? 2 : 1;
-  constexpr int TI = BM / 32, TJ = BN / 32;
-  constexpr int A_CH = BM * (NT_KB / 16) / NT_THREADS;  // 16-byte chunks per thread
-  constexpr int B_CH = BN * (NT_KB / 16) / NT_THREADS;
-  static_assert(A_CH >= 1 && B_CH >= 1, "tile too small for the loader");
-
-  __shared__ __attribute__((aligned(16))) uint8_t As[2][PL][BM * NT_ROW];
-  __shared__ __attribute__((aligned(16))) uint8_t Bs[2][PL][BN * NT_ROW];
-
-  const int tiles_m = (p.M + BM - 1) / BM, tiles_n = (p.N + BN - 1) / BN;
-  int tm, tn;
-  nt_tile_coords(tiles_m, tiles_n, tm, tn);
-  const int m0 = tm * BM, n0 = tn * BN;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave >> 1) * (BM / 2), wn0 = (wave & 1) * (BN / 2);
-  const int lr = lane & 15, lk = lane >> 4;
-
-  uint4 ra[PL][A_CH], rb[PL][B_CH];
-  auto load = [&](int kb) {
-#pragma unroll
-    for (int q = 0; q < A_CH; ++q) {
-      int ch = tid + q * NT_THREADS, row = ch >> 2, c16 = ch & 3;
-      size_t off = (size_t)(m0 + row) * p.lda + kb + c16 * 16;
-      ra[0][q] = *reinterpret_cast<const uint4*>(p.a0 + off);
-      if constexpr (PL == 2) ra[1][q] = *reinterpret_cast<const uint4*>(p.a1 + off);
-    }
-#pragma unroll
-    for (int q = 0; q < B_CH; ++q) {
-      int ch = tid + q * NT_THREADS, row = ch >> 2, c16 = ch & 3;
-      size_t off = (size_t)(n0 + row) * p.ldb + kb + c16 * 16;
-      rb[0][q] = *reinterpret_cast<const uint4*>(p.b0 + off);
-      if constexpr (PL == 2) rb[1][q] = *reinterpret_cast<const uint4*>(p.b1 + off);
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < A_CH; ++q) {
-      int ch = tid + q * NT_THREADS, row = ch >> 2, c16 = ch & 3;
-#pragma unroll
-      for (int pl = 0; pl < PL; ++pl) {
-        uint4 v = ra[pl][q];
-        if (KIND == NT_U8 || KIND == NT_QUANT) {
-          if (p.xor_a) {
-            v.x ^= 0x80808080u; v.y ^= 0x80808080u; v.z ^= 0x80808080u; v.w ^= 0x80808080u;
-          }
-        }
-        *reinterpret_cast<uint4*>(&As[buf][pl][row * NT_ROW + c16 * 16]) = v;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < B_CH; ++q) {
-      int ch = tid + q * NT_THREADS, row = ch >> 2, c16 = ch & 3;
-#pragma unroll
-      for (int pl = 0; pl < PL; ++pl)
-        *reinterpret_cast<uint4*>(&Bs[buf][pl][row * NT_ROW + c16 * 16]) = rb[pl][q];
-    }
-  };
-
-  using acc_t = typename std::conditional<HALF, f4, i4>::type;
-  acc_t acc0[TI][TJ], acc1[TI][TJ], acc2[TI][TJ];
-#pragma unroll
-  for (int i = 0; i < TI; ++i)
-#pragma unroll
-    for (int j = 0; j < TJ; ++j) {
-      acc0[i][j] = acc_t{0, 0, 0, 0};
-      acc1[i][j] = acc_t{0, 0, 0, 0};
-      acc2[i][j] = acc_t{0, 0, 0, 0};
-    }
-
-  const int nk = p.kbytes / NT_KB;
-  load(0);
-  store(0);
-  __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < nk;
-    if (more) load((kt + 1) * NT_KB);
-    uint4 af[PL][TI], bf[PL][TJ];
-#pragma unroll
-    for (int pl = 0; pl < PL; ++pl) {
-#pragma unroll
-      for (int i = 0; i < TI; ++i)
-        af[pl][i] = *reinterpret_cast<const uint4*>(&As[cur][pl][(wm0 + i * 16 + lr) * NT_ROW + lk * 16]);
-#pragma unroll
-      for (int j = 0; j < TJ; ++j)
-        bf[pl][j] = *reinterpret_cast<const uint4*>(&Bs[cur][pl][(wn0 + j * 16 + lr) * NT_ROW + lk * 16]);
-    }
-#pragma unroll
-    for (int i = 0; i < TI; ++i)
-#pragma unroll
-      for (int j = 0; j < TJ; ++j) {
-        if constexpr (HALF) {
-          acc0[i][j] = mfma_half<KIND>(af[0][i], bf[0][j], acc0[i][j]);
-        } else if constexpr (KIND == NT_I16) {
-          acc0[i][j] = mfma_i8(af[0][i], bf[0][j], acc0[i][j]);   // hi*hi
-          acc1[i][j] = mfma_i8(af[0][i], bf[1][j], acc1[i][j]);   // hi*lo'
-          acc1[i][j] = mfma_i8(af[1][i], bf[0][j], acc1[i][j]);   // lo'*hi
-          acc2[i][j] = mfma_i8(af[1][i], bf[1][j], acc2[i][j]);   // lo'*lo'
-        } else {
-          acc0[i][j] = mfma_i8(af[0][i], bf[0][j], acc0[i][j]);
-        }
-      }
-    if (more) store(cur ^ 1);
-    __syncthreads();
+// quantized_matmul (mod.rs:524-573): raw = acc + 128*csb (A was shifted by
+// -128); corrected = raw - azp*csb - bzp*rsa + k*azp*bzp (wrapping);
+// out = (scale_a*scale_b) * (float)corrected, written or added.
+struct EpiQuant {
+  float* c;
+  int M, N;
+  int accumulate;
+  const int32_t* colsum;
+  const int32_t* rowsum;
+  uint32_t azp, bzp, kab;
+  float scale;
+  __device__ __forceinline__ void operator()(int row, int col, int, int acc) const {
+    if (row >= M || col >= N) return;
+    const uint32_t cs = (uint32_t)colsum[col];
+    const uint32_t raw = (uint32_t)acc + 128u * cs;
+    const uint32_t v = raw - azp * cs - bzp * (uint32_t)rowsum[row] + kab;
+    float out = scale * (float)(int32_t)v;
+    float* dst = c + (size_t)row * N + col;
+    if (accumulate) out = *dst + out;
+    *dst = out;
   }
+};
 
-  // Epilogue: acc[i][j][r] is C[m0+wm0+16i+4*lk+r][n0+wn0+16j+lr].
-#pragma unroll
-  for (int i = 0; i < TI; ++i)
-#pragma unroll
-    for (int j = 0; j < TJ; ++j) {
-      const int col = n0 + wn0 + j * 16 + lr;
-      if (col >= p.N) continue;
-      uint32_t cs = 0;
-      if (KIND == NT_U8 || KIND == NT_QUANT || KIND == NT_I16) cs = (uint32_t)p.colsum[col];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm0 + i * 16 + lk * 4 + r;
-        if (row >= p.M) continue;
-        const size_t idx = (size_t)row * p.N + col;
-        if constexpr (HALF) {
-          float v = acc0[i][j][r];
-          float* dst = reinterpret_cast<float*>(p.c) + idx;
-          if (p.accumulate) v = *dst + v;
-          *dst = v;
-        } else if constexpr (KIND == NT_QUANT) {
-          // raw = sum a*b (u8 a); corrected per mod.rs:548-560
-          uint32_t raw = (uint32_t)acc0[i][j][r] + 128u * cs;
-          uint32_t azp = (uint32_t)p.azp, bzp = (uint32_t)p.bzp;
-          uint32_t v = raw - azp * cs - bzp * (uint32_t)p.rowsum[row] +
-                       (uint32_t)p.K * azp * bzp;
-          float out = p.scale * (float)(int32_t)v;
-          float* dst = reinterpret_cast<float*>(p.c) + idx;
-          if (p.accumulate) out = *dst + out;
-          *dst = out;
-        } else {
-          uint32_t v;
-          if constexpr (KIND == NT_I16) {
-            v = ((uint32_t)acc0[i][j][r] << 16) + ((uint32_t)acc1[i][j][r] << 8) +
-                (uint32_t)acc2[i][j][r] + 128u * (uint32_t)p.rowsum[row] + 128u * cs -
-                16384u * (uint32_t)p.K;
-          } else if constexpr (KIND == NT_U8) {
-            v = (uint32_t)acc0[i][j][r] + 128u * cs;
-          } else {
-            v = (uint32_t)acc0[i][j][r];
-          }
-          int32_t* dst = reinterpret_cast<int32_t*>(p.c) + idx;
-          if (p.accumulate) v += (uint32_t)*dst;
-          *dst = (int32_t)v;
-        }
-      }
-    }
-}
+struct EpiF32 {
+  float* c;
+  int M, N;
+  int accumulate;
+  __device__ __forceinline__ void operator()(int row, int col, int, float acc) const {
+    if (row >= M || col >= N) return;
+    float* dst = c + (size_t)row * N + col;
+    *dst = accumulate ? *dst + acc : acc;
+  }
+};
 
 // ---------------------------------------------------------------------
 // Prepasses
 // ---------------------------------------------------------------------
 
-// Bt[n][k] (n < Np, k < Kp) from row-major B (K x N), zero padded, with the
-// per-kind element map; optional wrapping column sums of the original B.
-//   kind 0: int8 copy     kind 1: i16 -> hi byte plane + (lo-128) plane
-//   kind 2: 16-bit copy
+// Bt planes from row-major B (K x N) of element E, tile 64 (k) x 64 (n),
+// zero padded to Np x Kp:
+//   MAP 0: byte copy      MAP 1: i16 -> hi byte plane + (lo - 128) plane
+//   MAP 2: 16-bit copy
+// colsum (optional) accumulates the wrapping column sums of B.
 template <typename E, int MAP>
-__global__ void pack_bt_kernel(const E* __restrict__ b, int K, int N, int Kp, int Np,
-                               uint8_t* out0, uint8_t* out1, int32_t* colsum) {
-  __shared__ E tile[32][33];
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int n0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
-  for (int r = ty; r < 32; r += 8) {
-    int k = k0 + r, n = n0 + tx;
-    tile[r][tx] = (k < K && n < N) ? b[(size_t)k * N + n] : E(0);
+__global__ __launch_bounds__(256) void pack_bt_kernel(const E* __restrict__ b, int K, int N,
+                                                      int Kp, int Np, uint8_t* out0,
+                                                      uint8_t* out1, int32_t* colsum) {
+  __shared__ E tile[64][64 + 4];
+  const int tid = threadIdx.x;
+  const int n0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  {
+    const int row = tid >> 2, c0 = (tid & 3) * 16;
+    const int k = k0 + row;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int n = n0 + c0 + e;
+      tile[row][c0 + e] = (k < K && n < N) ? b[(size_t)k * N + n] : E(0);
+    }
   }
   __syncthreads();
-  if (colsum && ty == 0 && n0 + tx < N) {
+  if (colsum && tid < 64 && n0 + tid < N) {
     uint32_t s = 0;
-    for (int r = 0; r < 32; ++r) s += (uint32_t)(int32_t)tile[r][tx];
-    atomicAdd(reinterpret_cast<unsigned int*>(colsum + n0 + tx), s);
+    for (int r = 0; r < 64; ++r) s += (uint32_t)(int32_t)tile[r][tid];
+    atomicAdd(reinterpret_cast<unsigned int*>(colsum + n0 + tid), s);
   }
-  for (int r = ty; r < 32; r += 8) {
-    int n = n0 + r, k = k0 + tx;
-    if (n >= Np || k >= Kp) continue;
-    E v = tile[tx][r];
-    bool real = (k < K && n < N);
-    if (MAP == 0) {
-      out0[(size_t)n * Kp + k] = (uint8_t)v;
-    } else if (MAP == 1) {
-      int x = (int)v;
-      out0[(size_t)n * Kp + k] = real ? (uint8_t)(int8_t)(x >> 8) : 0;
-      out1[(size_t)n * Kp + k] = real ? (uint8_t)(int8_t)((x & 255) - 128) : 0;
-    } else {
-      reinterpret_cast<uint16_t*>(out0)[(size_t)n * Kp + k] = (uint16_t)v;
+  const int nl = tid >> 2, kc = (tid & 3) * 16;
+  const int n = n0 + nl, kbase = k0 + kc;
+  if (n >= Np || kbase >= Kp) return;
+  if (MAP == 2) {
+    uint16_t v[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = (uint16_t)tile[kc + e][nl];
+    uint16_t* dst = reinterpret_cast<uint16_t*>(out0) + (size_t)n * Kp + kbase;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) dst[e] = v[e];
+  } else {
+    uint8_t v0[16], v1[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const bool real = (k0 + kc + e < K) && (n < N);
+      if (MAP == 0) {
+        v0[e] = (uint8_t)tile[kc + e][nl];
+      } else {
+        const int x = (int)tile[kc + e][nl];
+        v0[e] = real ? (uint8_t)(int8_t)(x >> 8) : 0;
+        v1[e] = real ? (uint8_t)(int8_t)((x & 255) - 128) : 0;
+      }
     }
+    *reinterpret_cast<u4v*>(out0 + (size_t)n * Kp + kbase) = __builtin_bit_cast(u4v, v0);
+    if (MAP == 1) *reinterpret_cast<u4v*>(out1 + (size_t)n * Kp + kbase) = __builtin_bit_cast(u4v, v1);
   }
 }
 
-// Ap[m][k] (m < Mp, k < Kp) from row-major A (M x K), same maps as above;
-// MAP 3: u8 -> u8 ^ 0x80 (padding stays 0).
+// Ap planes (Mp x Kp) from row-major A (M x K), same maps; MAP 3 flips the
+// sign bit of u8 (padding stays 0).
 template <typename E, int MAP>
 __global__ void pack_a_kernel(const E* __restrict__ a, int M, int K, int Mp, int Kp,
                               uint8_t* out0, uint8_t* out1) {
-  size_t total = (size_t)Mp * Kp;
+  const size_t total = (size_t)Mp * Kp;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (size_t)gridDim.x * blockDim.x) {
-    int m = (int)(i / Kp), k = (int)(i % Kp);
-    bool real = m < M && k < K;
-    E v = real ? a[(size_t)m * K + k] : E(0);
+    const int m = (int)(i / Kp), k = (int)(i % Kp);
+    const bool real = m < M && k < K;
+    const E v = real ? a[(size_t)m * K + k] : E(0);
     if (MAP == 0) {
       out0[i] = (uint8_t)v;
     } else if (MAP == 1) {
-      int x = (int)v;
+      const int x = (int)v;
       out0[i] = real ? (uint8_t)(int8_t)(x >> 8) : 0;
       out1[i] = real ? (uint8_t)(int8_t)((x & 255) - 128) : 0;
     } else if (MAP == 2) {
@@ -329,7 +205,6 @@ static unsigned grid_for(size_t count) {
 
 static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// Carves aligned sub-buffers out of one workspace allocation.
 struct Carver {
   size_t off = 0;
   size_t take(size_t bytes) {
@@ -339,110 +214,84 @@ struct Carver {
   }
 };
 
-template <int KIND, int BM, int BN>
-static void launch_nt(const NtArgs& p, hipStream_t st) {
-  size_t tiles = cdiv(p.M, BM) * cdiv(p.N, BN);
-  hipLaunchKernelGGL((gemm_nt_kernel<KIND, BM, BN>), dim3((unsigned)tiles), dim3(NT_THREADS), 0,
-                     st, p);
-}
+// ---------------------------------------------------------------------
+// Driver
+// ---------------------------------------------------------------------
 
-// One call of the NT family.  `a`, `b` are the caller's device operands.
-static int run_nt(int kind, size_t m, size_t n, size_t k, const void* a, const void* b, void* c,
-                  bool accumulate, uint8_t a_zp, float scale_a, int8_t b_zp, float scale_b,
-                  hipStream_t st) {
-  if (m == 0 || n == 0) return FERRAY_OK;
-  if (!c || (k && (!a || !b))) return FERRAY_ERR_INVALID;
-  if (m > (1u << 30) || n > (1u << 30) || k > (1u << 30)) return FERRAY_ERR_INVALID;
+template <class S>
+static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, const void* b,
+                        void* c, bool accumulate, uint8_t a_zp, float scale_a, int8_t b_zp,
+                        float scale_b, hipStream_t st) {
   const bool half = (kind == NT_BF16 || kind == NT_F16);
-  if (k == 0) {
-    if (!accumulate) {
-      hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(m * n)), dim3(256), 0, st,
-                         reinterpret_cast<uint32_t*>(c), m * n);
-      FG_CHECK(hipGetLastError());
-    }
-    return FERRAY_OK;
-  }
-  const size_t es = half ? 2 : (kind == NT_I16 ? 2 : 1);
-  const int BM = (kind == NT_I16) ? 64 : 128, BN = BM;
-  const size_t kelems_stage = NT_KB / (half ? 2 : 1);  // elements of k per stage (per plane)
-  const size_t Kp = align_up(k, kelems_stage);
-  const size_t Mp = align_up(m, BM), Np = align_up(n, BN);
-  const size_t pe = half ? 2 : 1;  // plane element bytes
+  const bool is_u8 = (kind == NT_U8 || kind == NT_QUANT);
+  const size_t es = (half || kind == NT_I16) ? 2 : 1;  // input element bytes
+  const size_t pe = half ? 2 : 1;                       // plane element bytes
+  const int planes = (kind == NT_I16) ? 2 : 1;
+  const size_t kel = S::KB / pe;                        // k elements per stage
+  const size_t Kp = align_up(k, kel);
+  const size_t Mp = align_up(m, S::BM), Np = align_up(n, S::BN);
   const bool need_colsum = (kind == NT_U8 || kind == NT_QUANT || kind == NT_I16);
   const bool need_rowsum = (kind == NT_QUANT || kind == NT_I16);
-  const bool direct_a = (kind != NT_I16) && (m % BM == 0) && ((k * es) % NT_KB == 0) &&
+  const bool direct_a = (kind != NT_I16) && (m % S::BM == 0) && ((k * es) % S::KB == 0) &&
                         ((uintptr_t)a % 16 == 0);
-  const int planes = (kind == NT_I16) ? 2 : 1;
 
   Carver cv;
-  size_t o_bt0 = cv.take(Np * Kp * pe);
-  size_t o_bt1 = planes == 2 ? cv.take(Np * Kp) : 0;
-  size_t o_cs = cv.take(Np * 4);
-  size_t o_rs = cv.take(Mp * 4);
-  size_t o_a0 = direct_a ? 0 : cv.take(Mp * Kp * pe);
-  size_t o_a1 = (!direct_a && planes == 2) ? cv.take(Mp * Kp) : 0;
+  const size_t o_bt0 = cv.take(Np * Kp * pe);
+  const size_t o_bt1 = planes == 2 ? cv.take(Np * Kp) : 0;
+  const size_t o_cs = cv.take(Np * 4);
+  const size_t o_rs = cv.take(Mp * 4);
+  const size_t o_a0 = direct_a ? 0 : cv.take(Mp * Kp * pe);
+  const size_t o_a1 = (!direct_a && planes == 2) ? cv.take(Mp * Kp) : 0;
   void* wsv = nullptr;
   int s = workspace(st, cv.off, &wsv);
   if (s) return s;
   uint8_t* ws = reinterpret_cast<uint8_t*>(wsv);
-
-  NtArgs p{};
-  p.b0 = ws + o_bt0;
-  p.b1 = planes == 2 ? ws + o_bt1 : nullptr;
-  p.ldb = Kp * pe;
-  p.kbytes = (int)(Kp * pe);
-  p.c = c;
-  p.M = (int)m;
-  p.N = (int)n;
-  p.K = (int)k;
-  p.accumulate = accumulate ? 1 : 0;
-  p.colsum = need_colsum ? reinterpret_cast<int32_t*>(ws + o_cs) : nullptr;
-  p.rowsum = need_rowsum ? reinterpret_cast<int32_t*>(ws + o_rs) : nullptr;
-  p.azp = (int32_t)a_zp;
-  p.bzp = (int32_t)b_zp;
-  p.scale = scale_a * scale_b;
+  int32_t* csp = need_colsum ? reinterpret_cast<int32_t*>(ws + o_cs) : nullptr;
+  int32_t* rsp = need_rowsum ? reinterpret_cast<int32_t*>(ws + o_rs) : nullptr;
 
   if (need_colsum) {
     hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(Np)), dim3(256), 0, st,
-                       reinterpret_cast<uint32_t*>(ws + o_cs), Np);
+                       reinterpret_cast<uint32_t*>(csp), Np);
   }
-  dim3 tgrid((unsigned)(Np / 32), (unsigned)(Kp / 32)), tblk(32, 8);
-  int32_t* csp = need_colsum ? reinterpret_cast<int32_t*>(ws + o_cs) : nullptr;
-  if (kind == NT_I16) {
-    hipLaunchKernelGGL((pack_bt_kernel<int16_t, 1>), tgrid, tblk, 0, st,
+  const dim3 tgrid((unsigned)(Np / 64), (unsigned)cdiv(Kp, 64));
+  if (kind == NT_I16)
+    hipLaunchKernelGGL((pack_bt_kernel<int16_t, 1>), tgrid, dim3(256), 0, st,
                        reinterpret_cast<const int16_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
                        ws + o_bt0, ws + o_bt1, csp);
-  } else if (half) {
-    hipLaunchKernelGGL((pack_bt_kernel<uint16_t, 2>), tgrid, tblk, 0, st,
+  else if (half)
+    hipLaunchKernelGGL((pack_bt_kernel<uint16_t, 2>), tgrid, dim3(256), 0, st,
                        reinterpret_cast<const uint16_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
                        ws + o_bt0, nullptr, nullptr);
-  } else {
-    hipLaunchKernelGGL((pack_bt_kernel<int8_t, 0>), tgrid, tblk, 0, st,
+  else
+    hipLaunchKernelGGL((pack_bt_kernel<int8_t, 0>), tgrid, dim3(256), 0, st,
                        reinterpret_cast<const int8_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
                        ws + o_bt0, nullptr, csp);
-  }
   FG_CHECK(hipGetLastError());
 
   if (need_rowsum) {
     if (kind == NT_I16)
       hipLaunchKernelGGL(rowsum_kernel<int16_t>, dim3((unsigned)m), dim3(256), 0, st,
-                         reinterpret_cast<const int16_t*>(a), (int)k,
-                         reinterpret_cast<int32_t*>(ws + o_rs));
+                         reinterpret_cast<const int16_t*>(a), (int)k, rsp);
     else
       hipLaunchKernelGGL(rowsum_kernel<uint8_t>, dim3((unsigned)m), dim3(256), 0, st,
-                         reinterpret_cast<const uint8_t*>(a), (int)k,
-                         reinterpret_cast<int32_t*>(ws + o_rs));
+                         reinterpret_cast<const uint8_t*>(a), (int)k, rsp);
     FG_CHECK(hipGetLastError());
   }
 
-  const bool is_u8 = (kind == NT_U8 || kind == NT_QUANT);
+  NtOperands p{};
+  p.bt = ws + o_bt0;
+  p.ldb = Kp * pe;
+  p.b_plane = 0;
+  p.a_plane = 0;
+  p.tiles_m = (int)(Mp / S::BM);
+  p.tiles_n = (int)(Np / S::BN);
+  bool xor_a = false;
   if (direct_a) {
-    p.a0 = reinterpret_cast<const uint8_t*>(a);
-    p.a1 = nullptr;
+    p.a = reinterpret_cast<const uint8_t*>(a);
     p.lda = k * es;
-    p.xor_a = is_u8 ? 1 : 0;
+    xor_a = is_u8;
   } else {
-    unsigned g = grid_for(Mp * Kp);
+    const unsigned g = grid_for(Mp * Kp);
     if (kind == NT_I16)
       hipLaunchKernelGGL((pack_a_kernel<int16_t, 1>), dim3(g), dim3(256), 0, st,
                          reinterpret_cast<const int16_t*>(a), (int)m, (int)k, (int)Mp, (int)Kp,
@@ -460,23 +309,138 @@ static int run_nt(int kind, size_t m, size_t n, size_t k, const void* a, const v
                          reinterpret_cast<const int8_t*>(a), (int)m, (int)k, (int)Mp, (int)Kp,
                          ws + o_a0, nullptr);
     FG_CHECK(hipGetLastError());
-    p.a0 = ws + o_a0;
-    p.a1 = planes == 2 ? ws + o_a1 : nullptr;
+    p.a = ws + o_a0;
     p.lda = Kp * pe;
-    p.xor_a = 0;
   }
 
-  switch (kind) {
-    case NT_I8S: launch_nt<NT_I8S, 128, 128>(p, st); break;
-    case NT_U8: launch_nt<NT_U8, 128, 128>(p, st); break;
-    case NT_QUANT: launch_nt<NT_QUANT, 128, 128>(p, st); break;
-    case NT_I16: launch_nt<NT_I16, 64, 64>(p, st); break;
-    case NT_BF16: launch_nt<NT_BF16, 128, 128>(p, st); break;
-    case NT_F16: launch_nt<NT_F16, 128, 128>(p, st); break;
-    default: return FERRAY_ERR_INVALID;
+  const int M = (int)m, N = (int)n;
+  // Integer paths reduce K in chunks so no i32 accumulator can overflow.
+  const size_t kb_total = Kp * pe;
+  const size_t chunk = half ? kb_total : align_up(NT_K_CHUNK, S::KB);
+  for (size_t kb0 = 0; kb0 < kb_total; kb0 += chunk) {
+    NtOperands q = p;
+    q.a = p.a + kb0;
+    q.bt = p.bt + kb0;
+    q.kbytes = (int)std::min(chunk, kb_total - kb0);
+    const bool first = kb0 == 0;
+    const int read_c = (!first || accumulate) ? 1 : 0;
+    int rc = FERRAY_OK;
+    if (half) {
+      EpiF32 e{reinterpret_cast<float*>(c), M, N, read_c};
+      rc = kind == NT_BF16 ? launch_mfma_nt<ELEM_BF16, S, false>(q, 1, e, st)
+                           : launch_mfma_nt<ELEM_F16, S, false>(q, 1, e, st);
+    } else if (kind == NT_QUANT) {
+      if (first) {
+        EpiQuant e{reinterpret_cast<float*>(c), M, N, accumulate ? 1 : 0, csp, rsp,
+                   (uint32_t)(int32_t)a_zp, (uint32_t)(int32_t)b_zp,
+                   (uint32_t)k * (uint32_t)(int32_t)a_zp * (uint32_t)(int32_t)b_zp,
+                   scale_a * scale_b};
+        rc = xor_a ? launch_mfma_nt<ELEM_I8, S, true>(q, 1, e, st)
+                   : launch_mfma_nt<ELEM_I8, S, false>(q, 1, e, st);
+      } else {
+        rc = FERRAY_ERR_INVALID;  // unreachable: run_nt routes k > 2^16 to run_quant_long
+      }
+    } else if (kind == NT_I16) {
+      // passes (A plane, B plane, shift): hi*hi<<16, hi*lo'<<8, lo'*hi<<8, lo'*lo'
+      const int pa[4] = {0, 0, 1, 1}, pb[4] = {0, 1, 0, 1}, sh[4] = {16, 8, 8, 0};
+      for (int ps = 0; ps < 4 && rc == FERRAY_OK; ++ps) {
+        NtOperands r = q;
+        r.a = q.a + (pa[ps] ? (ws + o_a1) - (ws + o_a0) : 0);
+        r.bt = q.bt + (pb[ps] ? (ws + o_bt1) - (ws + o_bt0) : 0);
+        const bool corr = first && ps == 0;
+        EpiI32 e{reinterpret_cast<int32_t*>(c), M, N, sh[ps], (ps > 0 || read_c) ? 1 : 0,
+                 corr ? csp : nullptr, corr ? rsp : nullptr,
+                 corr ? (uint32_t)(0u - 16384u * (uint32_t)k) : 0u};
+        rc = launch_mfma_nt<ELEM_I8, S, false>(r, 1, e, st);
+      }
+    } else {
+      EpiI32 e{reinterpret_cast<int32_t*>(c), M, N, 0, read_c,
+               (kind == NT_U8 && first) ? csp : nullptr, nullptr, 0u};
+      rc = xor_a ? launch_mfma_nt<ELEM_I8, S, true>(q, 1, e, st)
+                 : launch_mfma_nt<ELEM_I8, S, false>(q, 1, e, st);
+    }
+    if (rc) return rc;
   }
-  FG_CHECK(hipGetLastError());
   return FERRAY_OK;
+}
+
+// Wrapping column sums of a row-major K x N int8 matrix.
+__global__ void colsum_i8_kernel(const int8_t* __restrict__ b, int K, int N, int32_t* out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  uint32_t s = 0;
+  for (int kk = 0; kk < K; ++kk) s += (uint32_t)(int32_t)b[(size_t)kk * N + j];
+  out[j] = (int32_t)s;
+}
+
+// quantized_matmul epilogue over an exact raw u8 x i8 product.
+__global__ void quant_finish_kernel(const int32_t* __restrict__ raw, float* c, int M, int N,
+                                    const int32_t* colsum, const int32_t* rowsum, uint32_t azp,
+                                    uint32_t bzp, uint32_t kab, float scale, int accumulate) {
+  const size_t total = (size_t)M * N;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int row = (int)(i / N), col = (int)(i % N);
+    const uint32_t v = (uint32_t)raw[i] - azp * (uint32_t)colsum[col] -
+                       bzp * (uint32_t)rowsum[row] + kab;
+    const float out = scale * (float)(int32_t)v;
+    c[i] = accumulate ? c[i] + out : out;
+  }
+}
+
+static int run_nt(int kind, size_t m, size_t n, size_t k, const void* a, const void* b, void* c,
+                  bool accumulate, uint8_t a_zp, float scale_a, int8_t b_zp, float scale_b,
+                  hipStream_t st);
+
+// quantized_matmul with k > 2^16: the exact wrapping raw product is formed in
+// a stream-ordered temporary by the chunked u8 path, then finished
+// elementwise (mod.rs:507-573 in two steps, as the reference does).
+static int run_quant_long(size_t m, size_t n, size_t k, const void* a, const void* b, void* c,
+                          bool accumulate, uint8_t a_zp, float scale_a, int8_t b_zp,
+                          float scale_b, hipStream_t st) {
+  void* tmp = nullptr;
+  const size_t bytes = m * n * 4 + (m + n) * 4 + 512;
+  FG_CHECK(hipMallocAsync(&tmp, bytes, st));
+  int32_t* raw = reinterpret_cast<int32_t*>(tmp);
+  int32_t* rs = raw + m * n;
+  int32_t* cs = rs + m;
+  int rc = run_nt(NT_U8, m, n, k, a, b, raw, false, 0, 1.f, 0, 1.f, st);
+  if (rc == FERRAY_OK) {
+    hipLaunchKernelGGL(rowsum_kernel<uint8_t>, dim3((unsigned)m), dim3(256), 0, st,
+                       reinterpret_cast<const uint8_t*>(a), (int)k, rs);
+    hipLaunchKernelGGL(colsum_i8_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st,
+                       reinterpret_cast<const int8_t*>(b), (int)k, (int)n, cs);
+    const uint32_t azp = (uint32_t)(int32_t)a_zp, bzp = (uint32_t)(int32_t)b_zp;
+    hipLaunchKernelGGL(quant_finish_kernel, dim3(grid_for(m * n)), dim3(256), 0, st, raw,
+                       reinterpret_cast<float*>(c), (int)m, (int)n, cs, rs, azp, bzp,
+                       (uint32_t)k * azp * bzp, scale_a * scale_b, accumulate ? 1 : 0);
+    if (hipGetLastError() != hipSuccess) rc = FERRAY_ERR_HIP;
+  }
+  if (hipFreeAsync(tmp, st) != hipSuccess && rc == FERRAY_OK) rc = FERRAY_ERR_HIP;
+  return rc;
+}
+
+static int run_nt(int kind, size_t m, size_t n, size_t k, const void* a, const void* b, void* c,
+                  bool accumulate, uint8_t a_zp, float scale_a, int8_t b_zp, float scale_b,
+                  hipStream_t st) {
+  if (m == 0 || n == 0) return FERRAY_OK;
+  if (!c || (k && (!a || !b))) return FERRAY_ERR_INVALID;
+  if (m > (1u << 30) || n > (1u << 30) || k > (1u << 30)) return FERRAY_ERR_INVALID;
+  if (k == 0) {
+    if (!accumulate) {
+      hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(m * n)), dim3(256), 0, st,
+                         reinterpret_cast<uint32_t*>(c), m * n);
+      FG_CHECK(hipGetLastError());
+    }
+    return FERRAY_OK;
+  }
+  if (kind == NT_QUANT && k > NT_K_CHUNK) return run_quant_long(m, n, k, a, b, c, accumulate,
+                                                                a_zp, scale_a, b_zp, scale_b, st);
+  if (m >= 256 && n >= 256)
+    return run_nt_shape<NtBig>(kind, m, n, k, a, b, c, accumulate, a_zp, scale_a, b_zp, scale_b,
+                               st);
+  return run_nt_shape<NtSmall>(kind, m, n, k, a, b, c, accumulate, a_zp, scale_a, b_zp, scale_b,
+                               st);
 }
 
 }  // namespace fg
