@@ -14,6 +14,9 @@ Besides the headline the line carries:
                   on a bounded 4096^3 f64 sample;
   error         - normwise error of the 8192 result against a compensated
                   (double-double) dot product on sampled rows and columns;
+  split_precision - the Ozaki-scheme DGEMM on INT8 MFMA at the same size, per
+                  modulus count: time, speed-up over the native DGEMM above and
+                  sampled normwise error, for normal(0,1) and u*2^e inputs;
   sweep         - the other dtypes of the GEMM family at their config sizes
                   (f32 8192, c64/c32 4096, i8 8192, i16 4096, bf16 8192),
                   one timed number each (--no-sweep skips them).
@@ -86,6 +89,40 @@ def dgemm_error(fa, dA, dB, dC, n: int, samples: int = 16):
     return {"normwise_ab": float(err_f / (na * nb)), "rel_c": float(err_f / c_f),
             "max_abs": float(np.max(np.abs(d))),
             "sample": f"{samples} rows x {samples} cols vs compensated dot products"}
+
+
+def split_precision(fa, dA, dB, n: int, steps: int, native_ms: float, dist: str,
+                    moduli=(12, 13, 14, 15, 16), samples: int = 16):
+    """Ozaki-scheme DGEMM on INT8 MFMA: time and sampled error per modulus count."""
+    import oracle
+    a = dA.download(np.float64, n * n)
+    b = dB.download(np.float64, n * n)
+    rng = np.random.default_rng(321)
+    rows = np.sort(rng.choice(n, samples, replace=False))
+    cols = np.sort(rng.choice(n, samples, replace=False))
+    hi, lo = oracle.dd_gemm_f64_sampled(n, n, n, a, b, rows, cols)
+    na, nb = np.linalg.norm(a), np.linalg.norm(b)
+    frac = (n * n) / (samples * samples)
+    c_f = np.sqrt(np.sum(hi * hi) * frac)
+    dC = fa.DeviceBuffer(n * n * 8)
+    L = fa.lib()
+    out = {}
+    for nm in moduli:
+        def step():
+            st = L.ferray_dgemm_ozaki_dev(n, n, n, 1.0, dA.ptr, dB.ptr, 0.0, dC.ptr, nm, None)
+            if st != 0:
+                raise RuntimeError(f"ferray_dgemm_ozaki_dev status {st}")
+        ms = time_steps(fa, step, max(3, steps // 2), 1)
+        got = np.stack([dC.download(np.float64, n, offset_bytes=int(r) * n * 8) for r in rows])[:, cols]
+        d = (got - hi) - lo
+        err_f = np.sqrt(np.sum(d * d) * frac)
+        out[str(nm)] = {"ms": round(ms, 4), "tflops_equiv": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12, 2),
+                        "speedup_vs_native": round(native_ms / ms, 3),
+                        "normwise_ab": float(err_f / (na * nb)), "rel_c": float(err_f / c_f)}
+        log(f"ozaki[{dist}] moduli={nm}: {ms:.3f} ms ({native_ms / ms:.2f}x native), "
+            f"normwise {err_f / (na * nb):.2e}")
+    dC.free()
+    return out
 
 
 def cpu_baseline(n: int = 4096, reps: int = 3):
@@ -168,6 +205,7 @@ def main() -> int:
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-split", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the GEMM path runs on a single MI355X); "
@@ -214,6 +252,13 @@ def main() -> int:
     if not args.no_error:
         log("sampled error against compensated dot products")
         result["error"] = dgemm_error(fa, dA, dB, dC, n)
+    if not args.no_split:
+        sp = {"method": "integer-modular Ozaki scheme on INT8 MFMA; keys = number of moduli",
+              "normal": split_precision(fa, dA, dB, n, args.steps, ms, "normal")}
+        fa.fill(dA, n * n, fa.FILL_F64_WIDE, 3, -20, 20)
+        fa.fill(dB, n * n, fa.FILL_F64_WIDE, 4, -20, 20)
+        sp["wide_u_2e_pm20"] = split_precision(fa, dA, dB, n, args.steps, ms, "wide")
+        result["split_precision"] = sp
     for buf in (dA, dB, dC):
         buf.free()
     if not args.no_sweep:

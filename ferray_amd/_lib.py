@@ -49,11 +49,15 @@ SIGNATURES: dict[str, tuple] = {
     "ferray_quantized_matmul_dev": (_i, [_sz, _sz, _sz, _p, _u8, _f, _p, _i8, _f, _p, _b, _p]),
     "ferray_gemm_bf16_f32_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
     "ferray_gemm_f16_f32_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
+    "ferray_dgemm_ozaki": (_b, [_sz, _sz, _sz, _d, _p, _p, _d, _p, _i]),
+    "ferray_dgemm_ozaki_dev": (_i, [_sz, _sz, _sz, _d, _p, _p, _d, _p, _i, _p]),
+    "ferray_ozaki_max_moduli": (_i, []),
     "ferray_dev_malloc": (_i, [_sz, C.POINTER(_p)]),
     "ferray_dev_free": (_i, [_p]),
     "ferray_dev_upload": (_i, [_p, _p, _sz, _p]),
     "ferray_dev_download": (_i, [_p, _p, _sz, _p]),
     "ferray_dev_memset": (_i, [_p, _i, _sz, _p]),
+    "ferray_dev_copy": (_i, [_p, _p, _sz, _p]),
     "ferray_stream_sync": (_i, [_p]),
     "ferray_default_stream": (_p, []),
     "ferray_dev_fill": (_i, [_p, _sz, _i, _u64, _i64, _i64, _p]),

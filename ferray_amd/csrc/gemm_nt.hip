@@ -109,10 +109,21 @@ __global__ __launch_bounds__(256) void pack_bt_kernel(const E* __restrict__ b, i
   {
     const int row = tid >> 2, c0 = (tid & 3) * 16;
     const int k = k0 + row;
+    const E* src = b + (size_t)k * N + n0 + c0;
+    if (k < K && n0 + c0 + 16 <= N && ((uintptr_t)src % 16) == 0) {
+      // 16 in-range elements as 16-byte vector loads
+      E v[16];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int n = n0 + c0 + e;
-      tile[row][c0 + e] = (k < K && n < N) ? b[(size_t)k * N + n] : E(0);
+      for (int q = 0; q < (int)(16 * sizeof(E) / 16); ++q)
+        *reinterpret_cast<u4v*>(&v[q * 16 / sizeof(E)]) = reinterpret_cast<const u4v*>(src)[q];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) tile[row][c0 + e] = v[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int n = n0 + c0 + e;
+        tile[row][c0 + e] = (k < K && n < N) ? b[(size_t)k * N + n] : E(0);
+      }
     }
   }
   __syncthreads();

@@ -83,6 +83,17 @@ bool ferray_gemm_f64(size_t m, size_t n, size_t k, double alpha, const double* a
                        }));
 }
 
+bool ferray_dgemm_ozaki(size_t m, size_t n, size_t k, double alpha, const double* a,
+                        const double* b, double beta, double* c, int moduli) {
+  if (m == 0 || n == 0) return finish(FERRAY_OK);
+  return finish(staged(a, m * k * 8, b, k * n * 8, c, m * n * 8, beta != 0.0,
+                       [&](void* da, void* db, void* dc, hipStream_t st) {
+                         return ferray_dgemm_ozaki_dev(m, n, k, alpha, (const double*)da,
+                                                       (const double*)db, beta, (double*)dc,
+                                                       moduli, st);
+                       }));
+}
+
 bool ferray_gemm_f32(size_t m, size_t n, size_t k, float alpha, const float* a, const float* b,
                      float beta, float* c) {
   if (m == 0 || n == 0) return finish(FERRAY_OK);

@@ -1,0 +1,500 @@
+// Split-precision DGEMM on INT8 matrix cores: the integer-modular form of the
+// Ozaki scheme (Ozaki, Uchino, Imamura, "Ozaki Scheme II: A GEMM-oriented
+// emulation of floating-point matrix multiplication using an integer modular
+// technique", 2025).
+//
+//   1. Scaling (FP64).  Row i of A is scaled by 2^s_i and column j of B by
+//      2^t_j so that ||2^s_i a_i||_2 <= 2^g and ||2^t_j b_j||_2 <= 2^g, then
+//      rounded to integers A', B'.  With g = floor((log2 M - 2) / 2), where
+//      M = m_1 ... m_N is the product of the moduli, Cauchy-Schwarz gives
+//      |(A'B')_ij| <= M/4, so A'B' is recovered exactly from its residues.
+//   2. Residues (FP64 -> INT8).  For each modulus m_l (pairwise coprime,
+//      <= 256) the symmetric residues of A' and B' fit int8.
+//   3. N INT8 GEMMs on v_mfma_i32_32x32x32_i8 (exact: |sum| <= k*2^14 < 2^31
+//      for k <= 2^16 per chunk), each reduced mod m_l in the epilogue and
+//      stored as one byte.
+//   4. Reconstruction (FP64).  Chinese remaindering with the weights
+//      W_l = (M/m_l) * ((M/m_l)^-1 mod m_l) split into 32-bit pieces, so every
+//      partial sum is an exact double; the multiple of M is removed exactly,
+//      the result is evaluated from balanced 32-bit digits, and scaled back by
+//      2^-(s_i + t_j) before alpha/beta.
+// Rows or columns holding Inf/NaN produce NaN in the affected outputs.
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "mfma_nt.hpp"
+
+namespace fg {
+
+constexpr int OZ_MAX_MODULI = 16;
+// Pairwise coprime moduli, largest first (256 = 2^8, 255 = 3*5*17,
+// 253 = 11*23, 247 = 13*19, 217 = 7*31, the rest prime).
+static const int kModuli[OZ_MAX_MODULI] = {256, 255, 253, 251, 247, 241, 239, 233,
+                                           229, 227, 223, 217, 211, 199, 197, 193};
+
+struct OzConsts {
+  int nmod;
+  int m[OZ_MAX_MODULI];
+  float inv_m[OZ_MAX_MODULI];      // 1/m, for the residue reduction
+  unsigned c16[OZ_MAX_MODULI][3];  // 2^16, 2^32, 2^48 mod m
+  double w[OZ_MAX_MODULI][4];      // 32-bit pieces of the CRT weight W_l
+  double mp[4];                    // 32-bit pieces of M
+  double inv_M;                    // 1/M (approximate)
+};
+
+// ---------------------------------------------------------------------
+// Scaling
+// ---------------------------------------------------------------------
+
+// Running (exponent, scaled sum of squares) pair: the 2-norm of the values
+// seen so far is 2^E * sqrt(S), with S kept near 1 whatever the magnitudes
+// (the scaled-sum-of-squares update of LAPACK's dlassq).
+struct Ssq {
+  int E;
+  double S;
+};
+__device__ __forceinline__ Ssq ssq_add(Ssq a, double x) {
+  if (x == 0.0 || !isfinite(x)) return a;
+  const int ex = ilogb(x);
+  if (a.E == INT_MIN) return Ssq{ex, ldexp(x, -ex) * ldexp(x, -ex)};
+  if (ex > a.E) {
+    const double y = ldexp(x, -ex);
+    return Ssq{ex, fma(y, y, ldexp(a.S, 2 * (a.E - ex)))};
+  }
+  const double y = ldexp(x, -a.E);
+  return Ssq{a.E, fma(y, y, a.S)};
+}
+__device__ __forceinline__ Ssq ssq_merge(Ssq a, Ssq b) {
+  if (a.E == INT_MIN) return b;
+  if (b.E == INT_MIN) return a;
+  if (a.E >= b.E) return Ssq{a.E, a.S + ldexp(b.S, 2 * (b.E - a.E))};
+  return Ssq{b.E, b.S + ldexp(a.S, 2 * (a.E - b.E))};
+}
+// Shift that brings a vector of 2-norm 2^E sqrt(S) to at most 2^g.
+__device__ __forceinline__ int ssq_shift(Ssq a, int g) {
+  if (a.E == INT_MIN) return 0;
+  int ex;
+  frexp(sqrt(a.S), &ex);  // sqrt(S) <= 2^ex
+  return g - (a.E + ex);
+}
+
+// Row scales of A: s_i = g - e_i with ||a_i||_2 <= 2^e_i; rows holding
+// non-finite entries are flagged.  One block per row, one pass.
+__global__ __launch_bounds__(256) void oz_row_scale(const double* __restrict__ a, int K, int g,
+                                                    int* __restrict__ shift, int* __restrict__ bad) {
+  __shared__ int pe[256];
+  __shared__ double ps[256];
+  __shared__ int anybad;
+  const int row = blockIdx.x;
+  const double* r = a + (size_t)row * K;
+  if (threadIdx.x == 0) anybad = 0;
+  Ssq acc{INT_MIN, 0.0};
+  int nonfinite = 0;
+  for (int k = threadIdx.x; k < K; k += 256) {
+    const double x = r[k];
+    if (!isfinite(x)) nonfinite = 1;
+    acc = ssq_add(acc, x);
+  }
+  pe[threadIdx.x] = acc.E;
+  ps[threadIdx.x] = acc.S;
+  __syncthreads();
+  if (nonfinite) atomicOr(&anybad, 1);
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const Ssq m2 = ssq_merge(Ssq{pe[threadIdx.x], ps[threadIdx.x]},
+                               Ssq{pe[threadIdx.x + w], ps[threadIdx.x + w]});
+      pe[threadIdx.x] = m2.E;
+      ps[threadIdx.x] = m2.S;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    shift[row] = ssq_shift(Ssq{pe[0], ps[0]}, g);
+    bad[row] = anybad;
+  }
+}
+
+// Column statistics of B (K x N row-major): partial pairs per k-slice
+// (gridDim.y slices), combined by oz_col_finish.
+__global__ void oz_col_partial(const double* __restrict__ b, int K, int N, int kslice,
+                               int* __restrict__ pe, double* __restrict__ ps,
+                               int* __restrict__ bad) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const int k0 = blockIdx.y * kslice, k1 = min(K, k0 + kslice);
+  Ssq acc{INT_MIN, 0.0};
+  int nonfinite = 0;
+  for (int k = k0; k < k1; ++k) {
+    const double x = b[(size_t)k * N + j];
+    if (!isfinite(x)) nonfinite = 1;
+    acc = ssq_add(acc, x);
+  }
+  pe[(size_t)blockIdx.y * N + j] = acc.E;
+  ps[(size_t)blockIdx.y * N + j] = acc.S;
+  if (nonfinite) atomicOr(bad + j, 1);
+}
+
+__global__ void oz_col_finish(int N, int slices, int g, const int* __restrict__ pe,
+                              const double* __restrict__ ps, int* __restrict__ shift) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  Ssq acc{INT_MIN, 0.0};
+  for (int t = 0; t < slices; ++t)
+    acc = ssq_merge(acc, Ssq{pe[(size_t)t * N + j], ps[(size_t)t * N + j]});
+  shift[j] = ssq_shift(acc, g);
+}
+
+__global__ void oz_init(int* cbad, int N) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < N) cbad[j] = 0;
+}
+
+// ---------------------------------------------------------------------
+// Residues
+// ---------------------------------------------------------------------
+
+// Symmetric residue of the integer x (|x| < 2^63, held exactly in a double)
+// modulo c.m[l], from 16-bit limbs of |x|: s = d0 + d1*(2^16 mod m) +
+// d2*(2^32 mod m) + d3*(2^48 mod m) < 2^26, reduced with a float reciprocal
+// whose quotient is off by at most one.
+struct Limbs {
+  unsigned d0, d1, d2, d3;
+  bool neg;
+};
+__device__ __forceinline__ Limbs limbs_of(double x) {
+  Limbs L;
+  L.neg = x < 0.0;
+  const double ax = fabs(x);
+  const double hi = floor(ax * 0x1.0p-32);       // exact: < 2^31
+  const unsigned h = (unsigned)hi;
+  const unsigned l = (unsigned)fma(-hi, 0x1.0p32, ax);  // exact: < 2^32
+  L.d0 = l & 0xffff;
+  L.d1 = l >> 16;
+  L.d2 = h & 0xffff;
+  L.d3 = h >> 16;
+  return L;
+}
+__device__ __forceinline__ int sym_residue(const Limbs& L, const OzConsts& c, int l) {
+  const int m = c.m[l];
+  const unsigned s = L.d0 + L.d1 * c.c16[l][0] + L.d2 * c.c16[l][1] + L.d3 * c.c16[l][2];
+  int q = (int)((float)s * c.inv_m[l]);
+  int r = (int)s - q * m;
+  if (r < 0) r += m;
+  if (r >= m) r -= m;
+  if (L.neg && r) r = m - r;
+  if (r > (m - 1) / 2) r -= m;
+  return r;
+}
+
+// A planes: plane l is Mp x Kp int8, row-major.  Each thread covers 16
+// consecutive k of one row.
+__global__ __launch_bounds__(256) void oz_residues_a(const double* __restrict__ a, int M, int K,
+                                                     int Mp, int Kp, const int* __restrict__ shift,
+                                                     uint8_t* __restrict__ planes,
+                                                     size_t plane_bytes, OzConsts c) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t per_row = Kp / 16;
+  if (idx >= (size_t)Mp * per_row) return;
+  const int row = (int)(idx / per_row), k0 = (int)(idx % per_row) * 16;
+  Limbs L[16];
+  const int sh = row < M ? shift[row] : 0;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int k = k0 + e;
+    double x = (row < M && k < K) ? a[(size_t)row * K + k] : 0.0;
+    if (!isfinite(x)) x = 0.0;
+    L[e] = limbs_of(rint(ldexp(x, sh)));
+  }
+  uint8_t* dst = planes + (size_t)row * Kp + k0;
+  for (int l = 0; l < c.nmod; ++l) {
+    uint8_t v[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = (uint8_t)(int8_t)sym_residue(L[e], c, l);
+    *reinterpret_cast<u4v*>(dst + l * plane_bytes) = __builtin_bit_cast(u4v, v);
+  }
+}
+
+// B^T planes: plane l is Np x Kp int8 (row n holds column n of B).  Tile of
+// 64 (k) x 64 (n) through LDS; each thread covers 16 consecutive k of one n.
+__global__ __launch_bounds__(256) void oz_residues_bt(const double* __restrict__ b, int K, int N,
+                                                      int Kp, int Np, const int* __restrict__ shift,
+                                                      uint8_t* __restrict__ planes,
+                                                      size_t plane_bytes, OzConsts c) {
+  __shared__ double tile[64][65];
+  const int tid = threadIdx.x;
+  const int n0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    const int r = idx >> 6, cc = idx & 63;
+    const int k = k0 + r, n = n0 + cc;
+    tile[r][cc] = (k < K && n < N) ? b[(size_t)k * N + n] : 0.0;
+  }
+  __syncthreads();
+  const int nl = tid >> 2, kc = (tid & 3) * 16;
+  const int n = n0 + nl, kbase = k0 + kc;
+  if (n >= Np || kbase >= Kp) return;
+  const int sh = n < N ? shift[n] : 0;
+  Limbs L[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    double x = tile[kc + e][nl];
+    if (!isfinite(x)) x = 0.0;
+    L[e] = limbs_of(rint(ldexp(x, sh)));
+  }
+  uint8_t* dst = planes + (size_t)n * Kp + kbase;
+  for (int l = 0; l < c.nmod; ++l) {
+    uint8_t v[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = (uint8_t)(int8_t)sym_residue(L[e], c, l);
+    *reinterpret_cast<u4v*>(dst + l * plane_bytes) = __builtin_bit_cast(u4v, v);
+  }
+}
+
+// GEMM epilogue: residue of the exact i32 product modulo m_plane, as a byte.
+struct EpiResidue {
+  uint8_t* r;
+  size_t plane_bytes;
+  int ld, M, N;
+  int m[OZ_MAX_MODULI];
+  double inv_m[OZ_MAX_MODULI];
+  __device__ __forceinline__ void operator()(int row, int col, int z, int acc) const {
+    if (row >= M || col >= N) return;
+    const int mm = m[z];
+    // |acc| < 2^31: the product with the rounded reciprocal is within 2^-21
+    // of acc/m, so the floor is off by at most one and fixed below.
+    const double q = floor((double)acc * inv_m[z]);
+    int v = acc - (int)q * mm;
+    if (v < 0) v += mm;
+    if (v >= mm) v -= mm;
+    r[(size_t)z * plane_bytes + (size_t)row * ld + col] = (uint8_t)v;
+  }
+};
+
+// Reconstruction: C = alpha * 2^-(s_i+t_j) * CRT(r_1..r_N) + beta * C.  Each
+// thread finishes 8 consecutive columns of one row (residue planes have a
+// row pitch of ldr >= N, a multiple of 8, so one 8-byte load per plane).
+__device__ __forceinline__ double crt_value(const double (&r)[OZ_MAX_MODULI], const OzConsts& k) {
+  double S0 = 0, S1 = 0, S2 = 0, S3 = 0;
+  for (int l = 0; l < k.nmod; ++l) {
+    S0 = fma(r[l], k.w[l][0], S0);
+    S1 = fma(r[l], k.w[l][1], S1);
+    S2 = fma(r[l], k.w[l][2], S2);
+    S3 = fma(r[l], k.w[l][3], S3);
+  }
+  const double two32 = 4294967296.0;
+  const double xa = ((S3 * two32 + S2) * two32 + S1) * two32 + S0;
+  const double q = rint(xa * k.inv_M);
+  double D0 = fma(-q, k.mp[0], S0), D1 = fma(-q, k.mp[1], S1);
+  double D2 = fma(-q, k.mp[2], S2), D3 = fma(-q, k.mp[3], S3);
+  double t = rint(D0 * (1.0 / two32));
+  D0 = fma(-t, two32, D0);
+  D1 += t;
+  t = rint(D1 * (1.0 / two32));
+  D1 = fma(-t, two32, D1);
+  D2 += t;
+  t = rint(D2 * (1.0 / two32));
+  D2 = fma(-t, two32, D2);
+  D3 += t;
+  return ((D3 * two32 + D2) * two32 + D1) * two32 + D0;
+}
+
+__global__ __launch_bounds__(256) void oz_reconstruct(const uint8_t* __restrict__ res,
+                                                      size_t plane_bytes, int ldr, int M, int N,
+                                                      const int* __restrict__ sA,
+                                                      const int* __restrict__ tB,
+                                                      const int* __restrict__ badA,
+                                                      const int* __restrict__ badB, double alpha,
+                                                      double beta, double* __restrict__ c,
+                                                      OzConsts k) {
+  const int per_row = ldr / 8;
+  const size_t total = (size_t)M * per_row;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / per_row), j0 = (int)(idx % per_row) * 8;
+    if (j0 >= N) continue;
+    unsigned long long packed[OZ_MAX_MODULI];
+    const uint8_t* src = res + (size_t)i * ldr + j0;
+    for (int l = 0; l < k.nmod; ++l)
+      packed[l] = *reinterpret_cast<const unsigned long long*>(src + (size_t)l * plane_bytes);
+    const int si = sA[i], bi = badA[i];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int j = j0 + e;
+      if (j >= N) break;
+      double r[OZ_MAX_MODULI];
+#pragma unroll
+      for (int l = 0; l < OZ_MAX_MODULI; ++l)
+        r[l] = l < k.nmod ? (double)((packed[l] >> (8 * e)) & 0xff) : 0.0;
+      double v = ldexp(crt_value(r, k), -(si + tB[j]));
+      if (bi | badB[j]) v = __builtin_nan("");
+      v *= alpha;
+      double* dst = c + (size_t)i * N + j;
+      if (beta != 0.0) v = fma(beta, *dst, v);
+      *dst = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------
+
+typedef unsigned __int128 u128;
+
+static int make_consts(int nmod, OzConsts& c, int& gamma) {
+  if (nmod < 2 || nmod > OZ_MAX_MODULI) return FERRAY_ERR_INVALID;
+  std::memset(&c, 0, sizeof(c));
+  c.nmod = nmod;
+  u128 M = 1;
+  for (int l = 0; l < nmod; ++l) M *= (u128)kModuli[l];
+  int bits = 0;
+  for (u128 t = M; t > 1; t >>= 1) ++bits;  // floor(log2 M)
+  gamma = (bits - 2) / 2;
+  for (int l = 0; l < nmod; ++l) {
+    const int m = kModuli[l];
+    c.m[l] = m;
+    c.inv_m[l] = 1.0f / (float)m;
+    for (int t = 0; t < 3; ++t) {
+      u128 p = ((u128)1 << (16 * (t + 1))) % (u128)m;
+      c.c16[l][t] = (unsigned)p;
+    }
+    const u128 Ml = M / (u128)m;
+    const unsigned mm = (unsigned)(Ml % (u128)m);
+    unsigned inv = 0;
+    for (unsigned y = 1; y < (unsigned)m; ++y)
+      if ((mm * y) % (unsigned)m == 1) {
+        inv = y;
+        break;
+      }
+    if (!inv) return FERRAY_ERR_INVALID;
+    const u128 W = Ml * (u128)inv;  // < M since inv < m
+    for (int p = 0; p < 4; ++p) c.w[l][p] = (double)(unsigned)((W >> (32 * p)) & 0xffffffffu);
+  }
+  for (int p = 0; p < 4; ++p) c.mp[p] = (double)(unsigned)((M >> (32 * p)) & 0xffffffffu);
+  c.inv_M = 1.0 / (double)M;
+  return FERRAY_OK;
+}
+
+static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+template <class S>
+static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a,
+                     const double* b, double beta, double* c, int nmod, hipStream_t st) {
+  OzConsts cst;
+  int gamma = 0;
+  int s = make_consts(nmod, cst, gamma);
+  if (s) return s;
+  const size_t Kp = align_up(k, S::KB), Mp = align_up(m, S::BM), Np = align_up(n, S::BN);
+  const size_t ldr = align_up(n, 8);
+  const size_t a_plane = Mp * Kp, b_plane = Np * Kp, r_plane = align_up(m * ldr, 256);
+  const int kslice = 256;
+  const size_t slices = cdiv(k, kslice);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  const size_t o_pa = take(a_plane * nmod), o_pb = take(b_plane * nmod), o_r = take(r_plane * nmod);
+  const size_t o_sa = take(m * 4), o_ba = take(m * 4), o_tb = take(n * 4), o_bb = take(n * 4);
+  const size_t o_cm = take(slices * n * 4), o_cs = take(slices * n * 8);
+  void* wsv = nullptr;
+  if ((s = workspace(st, off, &wsv))) return s;
+  uint8_t* ws = reinterpret_cast<uint8_t*>(wsv);
+  int* sA = reinterpret_cast<int*>(ws + o_sa);
+  int* badA = reinterpret_cast<int*>(ws + o_ba);
+  int* tB = reinterpret_cast<int*>(ws + o_tb);
+  int* badB = reinterpret_cast<int*>(ws + o_bb);
+  int* pe = reinterpret_cast<int*>(ws + o_cm);
+  double* ps = reinterpret_cast<double*>(ws + o_cs);
+
+  hipLaunchKernelGGL(oz_row_scale, dim3((unsigned)m), dim3(256), 0, st, a, (int)k, gamma, sA, badA);
+  const unsigned gn = (unsigned)cdiv(n, 256);
+  hipLaunchKernelGGL(oz_init, dim3(gn), dim3(256), 0, st, badB, (int)n);
+  const dim3 cgrid(gn, (unsigned)slices);
+  hipLaunchKernelGGL(oz_col_partial, cgrid, dim3(256), 0, st, b, (int)k, (int)n, kslice, pe, ps,
+                     badB);
+  hipLaunchKernelGGL(oz_col_finish, dim3(gn), dim3(256), 0, st, (int)n, (int)slices, gamma, pe,
+                     ps, tB);
+  FG_CHECK(hipGetLastError());
+
+  const size_t athreads = Mp * (Kp / 16);
+  hipLaunchKernelGGL(oz_residues_a, dim3((unsigned)cdiv(athreads, 256)), dim3(256), 0, st, a,
+                     (int)m, (int)k, (int)Mp, (int)Kp, sA, ws + o_pa, a_plane, cst);
+  hipLaunchKernelGGL(oz_residues_bt, dim3((unsigned)(Np / 64), (unsigned)cdiv(Kp, 64)), dim3(256),
+                     0, st, b, (int)k, (int)n, (int)Kp, (int)Np, tB, ws + o_pb, b_plane, cst);
+  FG_CHECK(hipGetLastError());
+
+  NtOperands p{};
+  p.a = ws + o_pa;
+  p.bt = ws + o_pb;
+  p.a_plane = a_plane;
+  p.b_plane = b_plane;
+  p.lda = Kp;
+  p.ldb = Kp;
+  p.kbytes = (int)Kp;
+  p.tiles_m = (int)(Mp / S::BM);
+  p.tiles_n = (int)(Np / S::BN);
+  EpiResidue e{};
+  e.r = ws + o_r;
+  e.plane_bytes = r_plane;
+  e.ld = (int)ldr;
+  e.M = (int)m;
+  e.N = (int)n;
+  for (int l = 0; l < nmod; ++l) {
+    e.m[l] = kModuli[l];
+    e.inv_m[l] = 1.0 / (double)kModuli[l];
+  }
+  if ((s = launch_mfma_nt<ELEM_I8, S, false>(p, nmod, e, st))) return s;
+
+  size_t blocks = cdiv(m * (ldr / 8), 256);
+  if (blocks > 65536 * 4) blocks = 65536 * 4;
+  hipLaunchKernelGGL(oz_reconstruct, dim3((unsigned)blocks), dim3(256), 0, st, ws + o_r, r_plane,
+                     (int)ldr, (int)m, (int)n, sA, tB, badA, badB, alpha, beta, c, cst);
+  FG_CHECK(hipGetLastError());
+  return FERRAY_OK;
+}
+
+// k == 0 / empty handling shared with the native path's semantics.
+__global__ void oz_scale_c(double* c, size_t count, double beta) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (size_t)gridDim.x * blockDim.x)
+    c[i] = beta == 0.0 ? 0.0 : beta * c[i];
+}
+
+static int ozaki_dev(size_t m, size_t n, size_t k, double alpha, const double* a, const double* b,
+                     double beta, double* c, int nmod, hipStream_t st) {
+  if (m == 0 || n == 0) return FERRAY_OK;
+  if (!c || (k && (!a || !b))) return FERRAY_ERR_INVALID;
+  if (nmod == 0) nmod = OZ_MAX_MODULI;
+  if (nmod < 2 || nmod > OZ_MAX_MODULI) return FERRAY_ERR_INVALID;
+  // Each int8 product chunk must stay exact in i32: k * 128^2 < 2^31.
+  if (k > (1u << 16) || m > (1u << 30) || n > (1u << 30)) return FERRAY_ERR_INVALID;
+  if (k == 0) {
+    if (beta == 1.0) return FERRAY_OK;
+    size_t blocks = cdiv(m * n, 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(oz_scale_c, dim3((unsigned)blocks), dim3(256), 0, st, c, m * n, beta);
+    FG_CHECK(hipGetLastError());
+    return FERRAY_OK;
+  }
+  if (m >= 256 && n >= 256) return ozaki_run<NtBig>(m, n, k, alpha, a, b, beta, c, nmod, st);
+  return ozaki_run<NtSmall>(m, n, k, alpha, a, b, beta, c, nmod, st);
+}
+
+}  // namespace fg
+
+using namespace fg;
+
+extern "C" {
+
+int ferray_dgemm_ozaki_dev(size_t m, size_t n, size_t k, double alpha, const double* a,
+                           const double* b, double beta, double* c, int moduli, void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  return ozaki_dev(m, n, k, alpha, a, b, beta, c, moduli, resolve_stream(stream));
+}
+
+int ferray_ozaki_max_moduli(void) { return OZ_MAX_MODULI; }
+
+}  // extern "C"

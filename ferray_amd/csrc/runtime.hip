@@ -141,6 +141,19 @@ __global__ void fill_kernel(void* dst, size_t count, int kind, uint64_t seed,
   }
 }
 
+// Device-to-device copy, 16 bytes per thread where alignment allows.
+__global__ void copy_kernel(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src,
+                            size_t bytes) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t nvec = (((uintptr_t)dst | (uintptr_t)src) % 16 == 0) ? bytes / 16 : 0;
+  typedef unsigned u4_t __attribute__((ext_vector_type(4)));
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += stride)
+    reinterpret_cast<u4_t*>(dst)[i] = reinterpret_cast<const u4_t*>(src)[i];
+  for (size_t i = nvec * 16 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < bytes;
+       i += stride)
+    dst[i] = src[i];
+}
+
 }  // namespace fg
 
 using namespace fg;
@@ -203,6 +216,19 @@ int ferray_dev_download(void* dst, const void* src, size_t bytes, void* stream) 
   hipStream_t st = resolve_stream(stream);
   FG_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
   FG_CHECK(hipStreamSynchronize(st));
+  return FERRAY_OK;
+}
+
+int ferray_dev_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  if (!bytes) return FERRAY_OK;
+  if (!dst || !src) return FERRAY_ERR_INVALID;
+  size_t blocks = cdiv(bytes / 16 + 1, 256);
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, resolve_stream(stream),
+                     reinterpret_cast<uint8_t*>(dst), reinterpret_cast<const uint8_t*>(src), bytes);
+  FG_CHECK(hipGetLastError());
   return FERRAY_OK;
 }
 

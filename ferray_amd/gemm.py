@@ -21,7 +21,7 @@ from ._lib import check, lib
 __all__ = [
     "gpu_supports", "gemm_f64", "gemm_f32", "gemm_c64", "gemm_c32", "gemm_i8",
     "gemm_i8_signed", "gemm_i16", "quantized_matmul", "gemm_bf16_f32",
-    "gemm_f16_f32", "last_status", "DeviceBuffer", "fill", "sync",
+    "gemm_f16_f32", "dgemm_ozaki", "last_status", "DeviceBuffer", "fill", "sync",
     "FILL_F64_UNIFORM", "FILL_F64_NORMAL", "FILL_F64_WIDE", "FILL_F32_UNIFORM",
     "FILL_F32_NORMAL", "FILL_I8", "FILL_I16", "FILL_U8", "Event", "device_sync",
 ]
@@ -62,6 +62,15 @@ def gemm_f64(m, n, k, alpha, a, b, beta, c) -> bool:
     _need(a, np.float64, m * k, "a"); _need(b, np.float64, k * n, "b")
     _need(c, np.float64, m * n, "c", True)
     return bool(lib().ferray_gemm_f64(m, n, k, alpha, _ptr(a), _ptr(b), beta, _ptr(c)))
+
+
+def dgemm_ozaki(m, n, k, alpha, a, b, beta, c, moduli: int = 0) -> bool:
+    """Split-precision C = alpha*A@B + beta*C on INT8 matrix cores (integer
+    modular Ozaki scheme); same contract as gemm_f64 plus the modulus count."""
+    _need(a, np.float64, m * k, "a"); _need(b, np.float64, k * n, "b")
+    _need(c, np.float64, m * n, "c", True)
+    return bool(lib().ferray_dgemm_ozaki(m, n, k, alpha, _ptr(a), _ptr(b), beta, _ptr(c),
+                                         int(moduli)))
 
 
 def gemm_f32(m, n, k, alpha, a, b, beta, c) -> bool:

@@ -120,6 +120,26 @@ int ferray_gemm_f16_f32_dev(size_t m, size_t n, size_t k, const uint16_t *a,
                             const uint16_t *b, float *c, bool accumulate,
                             void *stream);
 
+/* ---------------- split-precision DGEMM ---------------- */
+
+/* C = alpha*A@B + beta*C in f64 computed on INT8 matrix cores by the
+ * integer-modular Ozaki scheme: A and B are scaled per row / per column by
+ * powers of two to integers, reduced modulo `moduli` pairwise-coprime
+ * moduli <= 256 (INT8 residues), multiplied exactly on INT8 MFMA, and
+ * recombined by Chinese remaindering in FP64.  moduli in [2, 16] (0 = 16)
+ * trades speed for accuracy: each modulus adds ~4 bits to the product's
+ * significand.  k <= 65536.  Rows/columns holding Inf or NaN give NaN.
+ * No counterpart in the reference (its DGEMM is gemm_f64, mod.rs:197-328);
+ * same argument order and meaning as ferray_gemm_f64 plus `moduli`. */
+bool ferray_dgemm_ozaki(size_t m, size_t n, size_t k, double alpha,
+                        const double *a, const double *b, double beta,
+                        double *c, int moduli);
+int ferray_dgemm_ozaki_dev(size_t m, size_t n, size_t k, double alpha,
+                           const double *a, const double *b, double beta,
+                           double *c, int moduli, void *stream);
+/* Largest supported number of moduli. */
+int ferray_ozaki_max_moduli(void);
+
 /* ---------------- device memory, streams, timing ---------------- */
 
 int ferray_dev_malloc(size_t bytes, void **out);
@@ -127,6 +147,8 @@ int ferray_dev_free(void *ptr);
 int ferray_dev_upload(void *dst, const void *src, size_t bytes, void *stream);
 int ferray_dev_download(void *dst, const void *src, size_t bytes, void *stream);
 int ferray_dev_memset(void *dst, int value, size_t bytes, void *stream);
+/* Device-to-device copy by a plain copy kernel (enqueued, not synchronised). */
+int ferray_dev_copy(void *dst, const void *src, size_t bytes, void *stream);
 int ferray_stream_sync(void *stream);
 /* The calling thread's library stream (what NULL resolves to). */
 void *ferray_default_stream(void);
