@@ -206,6 +206,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-split", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one 1 GiB device-to-device copy, the reference point "
+                         "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the GEMM path runs on a single MI355X); "
@@ -217,11 +220,22 @@ def main() -> int:
         print("no usable gfx950 device", file=sys.stderr)
         return 1
     n = args.size
+    L = fa.lib()
+    if args.copy_calibration:
+        gib = 1 << 30
+        src, dst = fa.DeviceBuffer(gib), fa.DeviceBuffer(gib)
+        fa.fill(src, gib // 8, fa.FILL_F64_UNIFORM, 5)
+        fa.device_sync()
+        if L.ferray_dev_copy(dst.ptr, src.ptr, gib, None) != 0:
+            raise RuntimeError("ferray_dev_copy failed")
+        fa.device_sync()
+        src.free()
+        dst.free()
+        log("copy calibration: 1 GiB read + 1 GiB written by copy_kernel")
     log(f"DGEMM {n}^3 f64: allocating and filling operands")
     dA, dB, dC = fa.DeviceBuffer(n * n * 8), fa.DeviceBuffer(n * n * 8), fa.DeviceBuffer(n * n * 8)
     fa.fill(dA, n * n, fa.FILL_F64_NORMAL, 1)
     fa.fill(dB, n * n, fa.FILL_F64_NORMAL, 2)
-    L = fa.lib()
 
     def step():
         st = L.ferray_gemm_f64_dev(n, n, n, 1.0, dA.ptr, dB.ptr, 0.0, dC.ptr, None)

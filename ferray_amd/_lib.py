@@ -52,6 +52,13 @@ SIGNATURES: dict[str, tuple] = {
     "ferray_dgemm_ozaki": (_b, [_sz, _sz, _sz, _d, _p, _p, _d, _p, _i]),
     "ferray_dgemm_ozaki_dev": (_i, [_sz, _sz, _sz, _d, _p, _p, _d, _p, _i, _p]),
     "ferray_ozaki_max_moduli": (_i, []),
+    "ferray_gemm_grouped_dev": (_i, [_p, _sz, _p]),
+    "ferray_grouped_plan_create": (_i, [_sz, _sz, C.POINTER(_p)]),
+    "ferray_grouped_plan_launch": (_i, [_p, _p, _sz, _p]),
+    "ferray_grouped_plan_sync": (_i, [_p]),
+    "ferray_grouped_plan_destroy": (_i, [_p]),
+    "ferray_set_tile_table": (_i, [_p, _sz]),
+    "ferray_get_tile_table": (_i, [_p, _sz]),
     "ferray_dev_malloc": (_i, [_sz, C.POINTER(_p)]),
     "ferray_dev_free": (_i, [_p]),
     "ferray_dev_upload": (_i, [_p, _p, _sz, _p]),
@@ -67,6 +74,17 @@ SIGNATURES: dict[str, tuple] = {
     "ferray_event_destroy": (_i, [_p]),
     "ferray_device_sync": (_i, []),
 }
+
+DTYPE_F64, DTYPE_F32, DTYPE_C64, DTYPE_C32, DTYPE_I8 = 0, 1, 2, 3, 4
+
+
+class GemmProblem(C.Structure):
+    """struct ferray_gemm_problem (include/ferray_gemm.h)."""
+    _fields_ = [("dtype", C.c_int), ("accumulate", C.c_int), ("m", C.c_size_t),
+                ("n", C.c_size_t), ("k", C.c_size_t), ("a", C.c_void_p), ("b", C.c_void_p),
+                ("c", C.c_void_p), ("alpha_re", C.c_double), ("alpha_im", C.c_double),
+                ("beta_re", C.c_double), ("beta_im", C.c_double)]
+
 
 _lock = threading.Lock()
 _lib: C.CDLL | None = None

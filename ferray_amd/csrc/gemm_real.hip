@@ -1,8 +1,14 @@
-// Real and complex floating-point GEMM on CDNA4 matrix cores.
+// Real and complex floating-point GEMM entry points (DGEMM, SGEMM, ZGEMM,
+// CGEMM) on CDNA4 matrix cores.
 //
 //   f64: v_mfma_f64_16x16x4_f64      f32: v_mfma_f32_16x16x4_f32
 //
-// One kernel template covers DGEMM, SGEMM, ZGEMM and CGEMM.  A complex
+// One kernel template covers the four.  Its main loop is written out here
+// rather than taken from real_tile.hpp: this form is the one measured
+// fastest with the current compiler (its register assignment keeps the f64
+// accumulator tuples 8-aligned).  Per output element it issues the same MFMA
+// k-sequence as real_tile(), which the grouped kernel uses, so both give
+// bit-identical results.  A complex
 // product C(m x n) = A(m x k) B(k x n) with interleaved [re, im] storage is
 // computed as the real product
 //     C_r(m x 2n) = A_r(m x 2k) . B_x(2k x 2n),
@@ -20,44 +26,11 @@
 // remapped so each XCD (blocks are dealt round-robin to the 8 XCDs) works on
 // a contiguous range of that order, which keeps the A and B panels a group
 // shares inside one XCD's L2.
-#include "common.hpp"
+#include "real_tile.hpp"
 
 namespace fg {
 
-template <typename T>
-struct Mfma16x16x4;
-// Operand layout of both: lane l supplies A[l%16][l/16] and B[l/16][l%16].
-// Result element r of lane l is C[row(l/16, r)][l%16]; the two instructions
-// lay the rows out differently (measured on gfx950).
-template <>
-struct Mfma16x16x4<double> {
-  using acc_t = d4;
-  static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
-};
-template <>
-struct Mfma16x16x4<float> {
-  using acc_t = f4;
-  static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
-};
-
-template <typename T>
-struct RealArgs {
-  const T* a;
-  const T* b;
-  T* c;
-  int M, N, K;  // real dimensions (complex: N = 2n, K = 2k)
-  T alpha_re, alpha_im, beta_re, beta_im;
-  int alpha_one;  // alpha == 1 (+0i): skip the scaling
-  int beta_zero;  // beta == 0: C is written without being read
-};
-
-constexpr int RB_M = 128, RB_N = 128, R_THREADS = 256, R_GROUP_M = 8;
+constexpr int RB_M = 128, RB_N = 128, R_THREADS = RT_THREADS, R_GROUP_M = 8;
 // k-depth of one pipeline stage: 8 for f64 (32 KiB of LDS, so two blocks fit
 // on a CU), 16 for f32.
 template <typename T>

@@ -1,0 +1,95 @@
+"""Grouped (batched, mixed-dtype) GEMM on device-resident operands.
+
+``grouped(problems)`` enqueues many independent problems in one persistent
+kernel launch; ``Plan`` keeps reusable launch state (descriptor buffers and a
+captured hipGraph) for callers with several batches in flight.  The tile
+shape of each problem comes from a table the autotuner writes
+(``ferray_amd/tile_table.json``), loaded when this module is imported.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+
+from ._lib import (DTYPE_C32, DTYPE_C64, DTYPE_F32, DTYPE_F64, DTYPE_I8, GemmProblem, check,
+                   lib)
+
+__all__ = ["GemmProblem", "Plan", "grouped", "problem", "load_tile_table", "tile_table",
+           "set_tile_table", "DTYPE_F64", "DTYPE_F32", "DTYPE_C64", "DTYPE_C32", "DTYPE_I8",
+           "TABLE_PATH"]
+
+TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tile_table.json")
+N_DTYPES, N_BUCKETS = 5, 8
+
+
+def problem(dtype: int, m: int, n: int, k: int, a: int, b: int, c: int, alpha=1.0, beta=0.0,
+            accumulate: bool = False) -> GemmProblem:
+    """Builds one problem descriptor; a, b, c are device addresses."""
+    alpha = complex(alpha)
+    beta = complex(beta)
+    return GemmProblem(dtype, int(bool(accumulate)), m, n, k, a, b, c, alpha.real, alpha.imag,
+                       beta.real, beta.imag)
+
+
+def _array(problems):
+    if isinstance(problems, C.Array):
+        return problems
+    arr = (GemmProblem * len(problems))()
+    for i, p in enumerate(problems):
+        arr[i] = p
+    return arr
+
+
+def grouped(problems, stream=None) -> None:
+    arr = _array(problems)
+    check(lib().ferray_gemm_grouped_dev(C.cast(arr, C.c_void_p), len(arr), stream), "grouped")
+
+
+class Plan:
+    """A reusable grouped launch (own descriptor buffers + captured graph)."""
+
+    def __init__(self, max_problems: int, max_tiles: int):
+        p = C.c_void_p()
+        check(lib().ferray_grouped_plan_create(max_problems, max_tiles, C.byref(p)), "plan_create")
+        self.ptr = p.value
+
+    def launch(self, problems, stream=None) -> None:
+        arr = _array(problems)
+        check(lib().ferray_grouped_plan_launch(self.ptr, C.cast(arr, C.c_void_p), len(arr),
+                                               stream), "plan_launch")
+
+    def sync(self) -> None:
+        check(lib().ferray_grouped_plan_sync(self.ptr), "plan_sync")
+
+    def close(self) -> None:
+        if self.ptr:
+            lib().ferray_grouped_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tile_table() -> list[list[int]]:
+    buf = (C.c_int * (N_DTYPES * N_BUCKETS))()
+    check(lib().ferray_get_tile_table(C.cast(buf, C.c_void_p), len(buf)), "get_tile_table")
+    return [list(buf[d * N_BUCKETS:(d + 1) * N_BUCKETS]) for d in range(N_DTYPES)]
+
+
+def set_tile_table(table) -> None:
+    flat = [int(v) for row in table for v in row]
+    buf = (C.c_int * len(flat))(*flat)
+    check(lib().ferray_set_tile_table(C.cast(buf, C.c_void_p), len(flat)), "set_tile_table")
+
+
+def load_tile_table(path: str = TABLE_PATH) -> bool:
+    """Loads the autotuner's table if present; returns whether it was applied."""
+    if not os.path.exists(path):
+        return False
+    with open(path) as f:
+        set_tile_table(json.load(f)["table"])
+    return True

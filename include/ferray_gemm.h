@@ -140,6 +140,56 @@ int ferray_dgemm_ozaki_dev(size_t m, size_t n, size_t k, double alpha,
 /* Largest supported number of moduli. */
 int ferray_ozaki_max_moduli(void);
 
+/* ---------------- grouped (batched, mixed-dtype) GEMM ---------------- */
+
+/* Element types of a grouped problem. */
+#define FERRAY_DTYPE_F64 0 /* gemm_f64: C = alpha*A@B + beta*C           */
+#define FERRAY_DTYPE_F32 1 /* gemm_f32                                   */
+#define FERRAY_DTYPE_C64 2 /* gemm_c64: interleaved complex, complex a/b */
+#define FERRAY_DTYPE_C32 3 /* gemm_c32                                   */
+#define FERRAY_DTYPE_I8 4  /* gemm_i8_signed: i8 x i8 -> i32 wrapping    */
+
+/* One independent GEMM of a grouped launch: device pointers, tightly packed
+ * row-major operands, the semantics of the per-dtype entry point (floating
+ * dtypes use alpha/beta, the real part only for real dtypes; I8 uses
+ * `accumulate`). */
+typedef struct ferray_gemm_problem {
+  int dtype;
+  int accumulate;
+  size_t m, n, k;
+  const void *a;
+  const void *b;
+  void *c;
+  double alpha_re, alpha_im, beta_re, beta_im;
+} ferray_gemm_problem;
+
+/* Many problems of uneven shape and mixed dtype in one persistent-kernel
+ * launch; each problem's result equals its single-problem entry point's bit
+ * for bit.  Uses a per-thread plan; returns once enqueued on `stream`.
+ * No counterpart in the reference: its only many-GEMM path is
+ * matmul_batched (products/mod.rs:579-659), same-shape problems on rayon. */
+int ferray_gemm_grouped_dev(const ferray_gemm_problem *problems, size_t count,
+                            void *stream);
+
+/* Reusable launch plans for callers with several batches in flight: each
+ * plan owns its descriptor buffers and a captured hipGraph of its launch
+ * (descriptor upload + persistent kernel) that is replayed per batch.
+ * plan_launch waits for the plan's previous launch before refilling it. */
+typedef struct ferray_grouped_plan ferray_grouped_plan;
+int ferray_grouped_plan_create(size_t max_problems, size_t max_tiles,
+                               ferray_grouped_plan **out);
+int ferray_grouped_plan_launch(ferray_grouped_plan *plan,
+                               const ferray_gemm_problem *problems,
+                               size_t count, void *stream);
+int ferray_grouped_plan_sync(ferray_grouped_plan *plan);
+int ferray_grouped_plan_destroy(ferray_grouped_plan *plan);
+
+/* Tile-shape table of the grouped kernel: entry [dtype*8 + b] is 0 (64x64
+ * tiles) or 1 (128x128) for problems with ceil(log2(max(m, n_real))) =
+ * clamp(., 4, 11) = b + 4; written by the package's autotuner. */
+int ferray_set_tile_table(const int *table, size_t len);
+int ferray_get_tile_table(int *table, size_t len);
+
 /* ---------------- device memory, streams, timing ---------------- */
 
 int ferray_dev_malloc(size_t bytes, void **out);
