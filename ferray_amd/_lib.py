@@ -59,6 +59,16 @@ SIGNATURES: dict[str, tuple] = {
     "ferray_grouped_plan_destroy": (_i, [_p]),
     "ferray_set_tile_table": (_i, [_p, _sz]),
     "ferray_get_tile_table": (_i, [_p, _sz]),
+    "ferray_server_create": (_i, [_p, C.POINTER(_p)]),
+    "ferray_server_destroy": (_i, [_p]),
+    "ferray_server_submit": (_i, [_p, _p, C.POINTER(_u64)]),
+    "ferray_server_poll": (_i, [_p, _u64, C.POINTER(_i)]),
+    "ferray_server_wait": (_i, [_p, _u64, C.POINTER(_d)]),
+    "ferray_server_result": (_i, [_p, _u64, _p, _sz]),
+    "ferray_server_result_dev": (_i, [_p, _u64, C.POINTER(_p)]),
+    "ferray_server_release": (_i, [_p, _u64]),
+    "ferray_server_stats_get": (_i, [_p, _p]),
+    "ferray_serve_trace": (_i, [_p, _p, _p, _sz, _p, C.POINTER(_d), _p]),
     "ferray_dev_malloc": (_i, [_sz, C.POINTER(_p)]),
     "ferray_dev_free": (_i, [_p]),
     "ferray_dev_upload": (_i, [_p, _p, _sz, _p]),
@@ -84,6 +94,30 @@ class GemmProblem(C.Structure):
                 ("n", C.c_size_t), ("k", C.c_size_t), ("a", C.c_void_p), ("b", C.c_void_p),
                 ("c", C.c_void_p), ("alpha_re", C.c_double), ("alpha_im", C.c_double),
                 ("beta_re", C.c_double), ("beta_im", C.c_double)]
+
+
+SERVE_GROUPED, SERVE_PER_REQUEST = 0, 1
+
+
+class ServerConfig(C.Structure):
+    """struct ferray_server_config (include/ferray_gemm.h)."""
+    _fields_ = [("hbm_bytes", C.c_size_t), ("max_resident", C.c_size_t),
+                ("max_batch", C.c_size_t), ("lanes", C.c_int), ("mode", C.c_int),
+                ("keep_results", C.c_int)]
+
+
+class Request(C.Structure):
+    """struct ferray_request (include/ferray_gemm.h)."""
+    _fields_ = [("dtype", C.c_int), ("operands_on_host", C.c_int), ("m", C.c_size_t),
+                ("n", C.c_size_t), ("k", C.c_size_t), ("alpha_re", C.c_double),
+                ("alpha_im", C.c_double), ("a", C.c_void_p), ("b", C.c_void_p)]
+
+
+class ServerStats(C.Structure):
+    """struct ferray_server_stats (include/ferray_gemm.h)."""
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("launches", C.c_uint64),
+                ("launched_requests", C.c_uint64), ("arena_bytes", C.c_size_t),
+                ("arena_peak", C.c_size_t)]
 
 
 _lock = threading.Lock()

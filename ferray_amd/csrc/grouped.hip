@@ -1,18 +1,20 @@
 // Grouped, persistent GEMM: one launch computes many independent problems of
 // uneven shape and mixed dtype (f64, f32, c64, c32, i8 x i8 -> i32).
 //
-// The host turns a problem list into a flat tile list (problems ordered by
-// decreasing work so the largest tiles start first), each problem tiled with
-// the shape a tunable table picks for its dtype and size (64x64 or 128x128).
-// A persistent grid of two blocks per CU walks the tile list with a fixed
-// stride, so which block computes which tile depends only on the list, and
-// every tile runs the same routine as the per-dtype entry points (no split
-// of k): a problem's result is bit-identical alone, in any mix, and through
-// its single-problem entry point.
+// The host orders the problems by decreasing work (so the largest tiles
+// start first), tiles each with the shape a tunable table picks for its dtype
+// and size (64x64 or 128x128) and gives each problem the index of its first
+// tile in one global numbering.  A persistent grid of up to two blocks per CU
+// walks the global tile numbers with a fixed stride and finds a tile's
+// problem by binary search over those first-tile indices.  Every tile runs
+// the same MFMA k-sequence as the per-dtype entry points (no split of k), so
+// a problem's result is bit-identical alone, in any mix, and through its
+// single-problem entry point.
 //
 // Plans own their pinned host and device descriptor buffers, so several
-// batches can be in flight on different streams; a plan's launch (descriptor
-// upload + kernel) is captured once into a hipGraph and replayed per batch.
+// batches can be in flight on different streams.  A plan's launch is a
+// hipGraph built once (descriptor upload -> kernel); per batch only the
+// upload size and the grid are updated before the graph is replayed.
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -27,7 +29,7 @@ typedef int i16g __attribute__((ext_vector_type(16)));
 
 struct GProblem {
   int dtype, cfg, tiles_n, accumulate;
-  int m, n, k, pad;
+  int m, n, k, tile_start;  // tile_start: global number of the first tile
   const void* a;
   const void* b;
   void* c;
@@ -204,18 +206,28 @@ __device__ __noinline__ void real_dispatch(const GProblem& P, int m0, int n0, ui
     real_tile<T, CPLX, true, BM, BN, BK>(r, m0, n0, As, Bs);
 }
 
-// Tile list: per tile, (problem index, tile index within the problem).
-// dev_count points at the number of tiles (read on the device so a captured
-// launch can be replayed for any batch).
+// Descriptor block: GHeader, then GProblem[nprob] ordered by tile_start.
+struct GHeader {
+  int nprob, ntiles;
+  int pad[62];
+};
+static_assert(sizeof(GHeader) == 256, "header size");
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
-void grouped_kernel(const GProblem* __restrict__ probs, const uint2* __restrict__ tiles,
-                    const int* __restrict__ dev_count) {
+void grouped_kernel(const uint8_t* __restrict__ desc) {
   __shared__ __attribute__((aligned(16))) uint8_t smem[G_LDS];
-  const int ntiles = *dev_count;
+  const GHeader* h = reinterpret_cast<const GHeader*>(desc);
+  const GProblem* probs = reinterpret_cast<const GProblem*>(desc + sizeof(GHeader));
+  const int np = h->nprob, ntiles = h->ntiles;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint2 tl = tiles[t];
-    const GProblem P = probs[tl.x];
-    const int tile = (int)tl.y;
+    int lo = 0, hi = np - 1;  // last problem whose first tile is <= t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (probs[mid].tile_start <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const GProblem P = probs[lo];
+    const int tile = t - P.tile_start;
     const int tm = tile / P.tiles_n, tn = tile % P.tiles_n;
     const int bm = P.cfg ? 128 : 64;
     const int m0 = tm * bm, n0 = tn * bm;
@@ -303,25 +315,24 @@ static int run_k0(const ferray_gemm_problem& p, hipStream_t st) {
 
 }  // namespace fg
 
-// A plan: pinned host staging + device buffers for up to `max_problems`
-// problems and `max_tiles` tiles, plus the captured graph of its launch.
+// A plan: pinned host staging + device copy of the descriptor block for up
+// to `max_problems` problems, and the graph of its launch.
 struct ferray_grouped_plan {
   size_t max_problems = 0, max_tiles = 0;
-  uint8_t* host = nullptr;   // [count int][pad][problems][tiles]
+  uint8_t* host = nullptr;  // GHeader + GProblem[max_problems]
   uint8_t* dev = nullptr;
   size_t bytes = 0;
-  hipStream_t capture_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphNode_t copy_node = nullptr, kernel_node = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t done = nullptr;
-  int grid = 0;
+  int max_grid = 0;
   bool pending = false;
 };
 
 namespace fg {
 
-static size_t plan_bytes(size_t probs, size_t tiles) {
-  return 256 + probs * sizeof(GProblem) + tiles * sizeof(uint2);
-}
+static size_t plan_bytes(size_t probs) { return sizeof(GHeader) + probs * sizeof(GProblem); }
 
 static int grid_blocks() {
   static int blocks = 0;
@@ -335,10 +346,11 @@ static int grid_blocks() {
   return blocks;
 }
 
-// Fills the plan's host staging with the tile list of `count` problems.
-// Returns the number of tiles (0 if every problem is empty).
-static int build_tiles(ferray_grouped_plan* plan, const ferray_gemm_problem* probs, size_t count,
-                       size_t* ntiles_out) {
+// Fills the plan's host staging for `count` problems; returns the number of
+// tiles through *ntiles_out and the number of descriptor bytes to upload
+// through *bytes_out.
+static int build_desc(ferray_grouped_plan* plan, const ferray_gemm_problem* probs, size_t count,
+                      size_t* ntiles_out, size_t* bytes_out) {
   std::vector<std::pair<size_t, size_t>> order;  // (work, index)
   order.reserve(count);
   for (size_t i = 0; i < count; ++i) {
@@ -347,30 +359,24 @@ static int build_tiles(ferray_grouped_plan* plan, const ferray_gemm_problem* pro
     const size_t f = (p.dtype == FERRAY_DTYPE_C64 || p.dtype == FERRAY_DTYPE_C32) ? 4 : 1;
     order.push_back({p.m * p.n * p.k * f, i});
   }
+  if (order.size() > plan->max_problems) return FERRAY_ERR_INVALID;
   std::stable_sort(order.begin(), order.end(),
                    [](const auto& x, const auto& y) { return x.first > y.first; });
+  GProblem* gp = reinterpret_cast<GProblem*>(plan->host + sizeof(GHeader));
   size_t total = 0;
-  for (auto& o : order) {
-    const ferray_gemm_problem& p = probs[o.second];
-    int tn;
-    total += problem_tiles(p.dtype, grouped_cfg(p.dtype, p.m, p.n), p.m, p.n, &tn);
-  }
-  if (order.size() > plan->max_problems || total > plan->max_tiles) return FERRAY_ERR_INVALID;
-  if (total > 0x7fffffff) return FERRAY_ERR_INVALID;
-  GProblem* gp = reinterpret_cast<GProblem*>(plan->host + 256);
-  uint2* tl = reinterpret_cast<uint2*>(plan->host + 256 + plan->max_problems * sizeof(GProblem));
-  size_t t = 0;
   for (size_t j = 0; j < order.size(); ++j) {
     const ferray_gemm_problem& p = probs[order[j].second];
     GProblem& g = gp[j];
     g.dtype = p.dtype;
     g.cfg = grouped_cfg(p.dtype, p.m, p.n);
     const size_t nt = problem_tiles(p.dtype, g.cfg, p.m, p.n, &g.tiles_n);
+    if (total + nt > plan->max_tiles || total + nt > 0x7fffffff) return FERRAY_ERR_INVALID;
+    g.tile_start = (int)total;
+    total += nt;
     g.accumulate = p.accumulate;
     g.m = (int)p.m;
     g.n = (int)p.n;
     g.k = (int)p.k;
-    g.pad = 0;
     g.a = p.a;
     g.b = p.b;
     g.c = p.c;
@@ -378,28 +384,47 @@ static int build_tiles(ferray_grouped_plan* plan, const ferray_gemm_problem* pro
     g.ai = p.alpha_im;
     g.br = p.beta_re;
     g.bi = p.beta_im;
-    for (size_t q = 0; q < nt; ++q) tl[t++] = uint2{(unsigned)j, (unsigned)q};
   }
-  *reinterpret_cast<int*>(plan->host) = (int)total;
+  GHeader* h = reinterpret_cast<GHeader*>(plan->host);
+  h->nprob = (int)order.size();
+  h->ntiles = (int)total;
   *ntiles_out = total;
+  *bytes_out = plan_bytes(order.size());
   return FERRAY_OK;
 }
 
-static int plan_capture(ferray_grouped_plan* plan) {
-  // Graph: upload the whole staging area, then the persistent kernel.
-  hipStream_t cs = plan->capture_stream;
-  FG_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-  hipMemcpyAsync(plan->dev, plan->host, plan->bytes, hipMemcpyHostToDevice, cs);
-  const GProblem* dp = reinterpret_cast<const GProblem*>(plan->dev + 256);
-  const uint2* dt = reinterpret_cast<const uint2*>(plan->dev + 256 + plan->max_problems * sizeof(GProblem));
-  const int* dc = reinterpret_cast<const int*>(plan->dev);
-  hipLaunchKernelGGL(grouped_kernel, dim3(plan->grid), dim3(256), 0, cs, dp, dt, dc);
-  hipGraph_t graph = nullptr;
-  hipError_t e = hipStreamEndCapture(cs, &graph);
-  if (e != hipSuccess) return map_hip_error(e);
-  e = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  if (e != hipSuccess) return map_hip_error(e);
+// The launch graph: descriptor upload (pinned host -> device), then the
+// persistent kernel.  Built once per plan; launches update the upload size
+// and the grid.
+static int plan_graph(ferray_grouped_plan* plan) {
+  FG_CHECK(hipGraphCreate(&plan->graph, 0));
+  FG_CHECK(hipGraphAddMemcpyNode1D(&plan->copy_node, plan->graph, nullptr, 0, plan->dev,
+                                   plan->host, plan->bytes, hipMemcpyHostToDevice));
+  hipKernelNodeParams kp = {};
+  void* args[] = {&plan->dev};
+  kp.func = reinterpret_cast<void*>(grouped_kernel);
+  kp.gridDim = dim3(plan->max_grid);
+  kp.blockDim = dim3(256);
+  kp.sharedMemBytes = 0;
+  kp.kernelParams = args;
+  kp.extra = nullptr;
+  FG_CHECK(hipGraphAddKernelNode(&plan->kernel_node, plan->graph, &plan->copy_node, 1, &kp));
+  FG_CHECK(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+  return FERRAY_OK;
+}
+
+static int plan_update(ferray_grouped_plan* plan, size_t bytes, int grid) {
+  FG_CHECK(hipGraphExecMemcpyNodeSetParams1D(plan->exec, plan->copy_node, plan->dev, plan->host,
+                                             bytes, hipMemcpyHostToDevice));
+  hipKernelNodeParams kp = {};
+  void* args[] = {&plan->dev};
+  kp.func = reinterpret_cast<void*>(grouped_kernel);
+  kp.gridDim = dim3(grid);
+  kp.blockDim = dim3(256);
+  kp.sharedMemBytes = 0;
+  kp.kernelParams = args;
+  kp.extra = nullptr;
+  FG_CHECK(hipGraphExecKernelNodeSetParams(plan->exec, plan->kernel_node, &kp));
   return FERRAY_OK;
 }
 
@@ -438,18 +463,17 @@ int ferray_grouped_plan_create(size_t max_problems, size_t max_tiles, ferray_gro
   if (!p) return FERRAY_ERR_OOM;
   p->max_problems = max_problems;
   p->max_tiles = max_tiles;
-  p->bytes = plan_bytes(max_problems, max_tiles);
-  p->grid = grid_blocks();
+  p->bytes = plan_bytes(max_problems);
+  p->max_grid = grid_blocks();
   hipError_t e = hipHostMalloc((void**)&p->host, p->bytes, hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc((void**)&p->dev, p->bytes);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->done, hipEventDisableTiming);
   if (e != hipSuccess) {
     ferray_grouped_plan_destroy(p);
     return map_hip_error(e);
   }
-  std::memset(p->host, 0, 256);
-  if ((s = plan_capture(p))) {
+  std::memset(p->host, 0, p->bytes);
+  if ((s = plan_graph(p))) {
     ferray_grouped_plan_destroy(p);
     return s;
   }
@@ -461,8 +485,8 @@ int ferray_grouped_plan_destroy(ferray_grouped_plan* p) {
   if (!p) return FERRAY_OK;
   if (p->pending && p->done) (void)hipEventSynchronize(p->done);
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->done) (void)hipEventDestroy(p->done);
-  if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
   if (p->dev) (void)hipFree(p->dev);
   if (p->host) (void)hipHostFree(p->host);
   delete p;
@@ -483,9 +507,10 @@ int ferray_grouped_plan_launch(ferray_grouped_plan* p, const ferray_gemm_problem
   for (size_t i = 0; i < count; ++i)
     if (problems[i].m && problems[i].n && !problems[i].k)
       if ((s = run_k0(problems[i], st))) return s;
-  size_t ntiles = 0;
-  if ((s = build_tiles(p, problems, count, &ntiles))) return s;
+  size_t ntiles = 0, bytes = 0;
+  if ((s = build_desc(p, problems, count, &ntiles, &bytes))) return s;
   if (ntiles == 0) return FERRAY_OK;
+  if ((s = plan_update(p, bytes, (int)std::min<size_t>(ntiles, p->max_grid)))) return s;
   FG_CHECK(hipGraphLaunch(p->exec, st));
   FG_CHECK(hipEventRecord(p->done, st));
   p->pending = true;

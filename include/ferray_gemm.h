@@ -190,6 +190,77 @@ int ferray_grouped_plan_destroy(ferray_grouped_plan *plan);
 int ferray_set_tile_table(const int *table, size_t len);
 int ferray_get_tile_table(int *table, size_t len);
 
+/* ---------------- serving loop ----------------
+ * No counterpart in the reference (its GEMM calls are synchronous library
+ * calls).  A server owns an HBM arena, a worker thread and `lanes` launch
+ * lanes (a stream and a grouped plan each).  Requests are admitted in
+ * arrival order while their footprint (C, plus A and B when the operands
+ * come from the host) fits in the arena and fewer than `max_resident` are
+ * resident.  In GROUPED mode every idle lane takes up to `max_batch`
+ * admitted requests into one grouped launch, so requests that arrive while
+ * others run are batched into the next launch on another lane and overlap on
+ * the device (continuous batching).  PER_REQUEST mode launches each admitted
+ * request through its single-problem entry point at once (the baseline).
+ * Each request computes C = alpha * A.B (C in the arena); its result is
+ * bit-identical in either mode and alone or in any mix. */
+#define FERRAY_SERVE_GROUPED 0
+#define FERRAY_SERVE_PER_REQUEST 1
+
+typedef struct ferray_server ferray_server;
+
+typedef struct ferray_server_config {
+  size_t hbm_bytes;    /* arena size (admission budget), > 0 */
+  size_t max_resident; /* admitted-and-unreleased requests at most, > 0 */
+  size_t max_batch;    /* requests per grouped launch at most, > 0 */
+  int lanes;           /* launch lanes (streams), 1..16 */
+  int mode;            /* FERRAY_SERVE_GROUPED or FERRAY_SERVE_PER_REQUEST */
+  int keep_results;    /* 1: C stays until ferray_server_result/release;
+                          0: C is dropped on completion (benchmarks) */
+} ferray_server_config;
+
+typedef struct ferray_request {
+  int dtype;            /* FERRAY_DTYPE_*; I8 means i8 x i8 -> i32 */
+  int operands_on_host; /* 0: a, b are device-resident; 1: host memory that
+                           is copied in (must stay valid until completion) */
+  size_t m, n, k;
+  double alpha_re, alpha_im; /* I8 ignores alpha */
+  const void *a;
+  const void *b;
+} ferray_request;
+
+typedef struct ferray_server_stats {
+  uint64_t submitted, completed, launches, launched_requests;
+  size_t arena_bytes, arena_peak;
+} ferray_server_stats;
+
+int ferray_server_create(const ferray_server_config *cfg, ferray_server **out);
+/* Drains every submitted request, then frees the server. */
+int ferray_server_destroy(ferray_server *server);
+int ferray_server_submit(ferray_server *server, const ferray_request *req,
+                         uint64_t *ticket);
+/* state: 0 queued, 1 admitted, 2 running, 3 done. */
+int ferray_server_poll(ferray_server *server, uint64_t ticket, int *state);
+/* Blocks until done; latency = completion - submission, milliseconds. */
+int ferray_server_wait(ferray_server *server, uint64_t ticket,
+                       double *latency_ms);
+/* keep_results servers: copies C to host memory (`bytes` must equal C's
+ * size) and releases the request. */
+int ferray_server_result(ferray_server *server, uint64_t ticket, void *c_host,
+                         size_t bytes);
+/* keep_results servers: device address of C, valid until release. */
+int ferray_server_result_dev(ferray_server *server, uint64_t ticket,
+                             const void **c_dev);
+int ferray_server_release(ferray_server *server, uint64_t ticket);
+int ferray_server_stats_get(ferray_server *server, ferray_server_stats *out);
+
+/* Replays a trace through a fresh server (keep_results forced to 0): request
+ * i is submitted arrival_s[i] seconds after the start (non-decreasing), and
+ * latency_ms[i] = completion - scheduled arrival.  *wall_s = last completion
+ * - start.  Returns the first failing status, if any. */
+int ferray_serve_trace(const ferray_server_config *cfg, const ferray_request *reqs,
+                       const double *arrival_s, size_t count, double *latency_ms,
+                       double *wall_s, ferray_server_stats *stats);
+
 /* ---------------- device memory, streams, timing ---------------- */
 
 int ferray_dev_malloc(size_t bytes, void **out);
