@@ -93,3 +93,9 @@ def load_tile_table(path: str = TABLE_PATH) -> bool:
     with open(path) as f:
         set_tile_table(json.load(f)["table"])
     return True
+
+
+try:  # the autotuner's table, when one has been recorded for this device
+    load_tile_table()
+except (OSError, ValueError, KeyError):
+    pass
