@@ -59,6 +59,8 @@ SIGNATURES: dict[str, tuple] = {
     "ferray_grouped_plan_destroy": (_i, [_p]),
     "ferray_set_tile_table": (_i, [_p, _sz]),
     "ferray_get_tile_table": (_i, [_p, _sz]),
+    "ferray_matmul_batched": (_b, [_i, _sz, _sz, _sz, _sz, _sz, _sz, _p, _p, _p]),
+    "ferray_matmul_batched_dev": (_i, [_i, _sz, _sz, _sz, _sz, _sz, _sz, _p, _p, _p, _p]),
     "ferray_server_create": (_i, [_p, C.POINTER(_p)]),
     "ferray_server_destroy": (_i, [_p]),
     "ferray_server_submit": (_i, [_p, _p, C.POINTER(_u64)]),

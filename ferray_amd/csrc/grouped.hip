@@ -553,4 +553,34 @@ int ferray_gemm_grouped_dev(const ferray_gemm_problem* problems, size_t count, v
   return ferray_grouped_plan_launch(h.plan, problems, count, stream);
 }
 
+int ferray_matmul_batched_dev(int dtype, size_t batch, size_t a_batch, size_t b_batch, size_t m,
+                              size_t k, size_t n, const void* a, const void* b, void* c,
+                              void* stream) {
+  if (dtype < FERRAY_DTYPE_F64 || dtype > FERRAY_DTYPE_I8) return FERRAY_ERR_INVALID;
+  if (batch == 0 || m == 0 || n == 0) return FERRAY_OK;
+  if (a_batch == 0 || b_batch == 0 || !c || (k && (!a || !b))) return FERRAY_ERR_INVALID;
+  const size_t e = (dtype == FERRAY_DTYPE_F64 || dtype == FERRAY_DTYPE_C32) ? 8
+                   : dtype == FERRAY_DTYPE_F32                              ? 4
+                   : dtype == FERRAY_DTYPE_C64                              ? 16
+                                                                            : 1;
+  const size_t ce = dtype == FERRAY_DTYPE_I8 ? 4 : e;
+  std::vector<ferray_gemm_problem> probs(batch);
+  for (size_t i = 0; i < batch; ++i) {
+    ferray_gemm_problem& p = probs[i];
+    p.dtype = dtype;
+    p.accumulate = 0;
+    p.m = m;
+    p.n = n;
+    p.k = k;
+    p.a = a ? static_cast<const uint8_t*>(a) + (i % a_batch) * m * k * e : nullptr;
+    p.b = b ? static_cast<const uint8_t*>(b) + (i % b_batch) * k * n * e : nullptr;
+    p.c = static_cast<uint8_t*>(c) + i * m * n * ce;
+    p.alpha_re = 1.0;
+    p.alpha_im = 0.0;
+    p.beta_re = 0.0;
+    p.beta_im = 0.0;
+  }
+  return ferray_gemm_grouped_dev(probs.data(), batch, stream);
+}
+
 }  // extern "C"

@@ -2,6 +2,7 @@
 // staged into per-thread device buffers, the device entry point runs on the
 // thread's stream, and the call returns once C is back in host memory.
 // Any failure returns false with the reason in ferray_gemm_last_status().
+#include <algorithm>
 #include <cstring>
 
 #include "common.hpp"
@@ -80,6 +81,24 @@ bool ferray_gemm_f64(size_t m, size_t n, size_t k, double alpha, const double* a
                        [&](void* da, void* db, void* dc, hipStream_t st) {
                          return ferray_gemm_f64_dev(m, n, k, alpha, (const double*)da,
                                                     (const double*)db, beta, (double*)dc, st);
+                       }));
+}
+
+bool ferray_matmul_batched(int dtype, size_t batch, size_t a_batch, size_t b_batch, size_t m,
+                           size_t k, size_t n, const void* a, const void* b, void* c) {
+  if (dtype < FERRAY_DTYPE_F64 || dtype > FERRAY_DTYPE_I8 || a_batch == 0 || b_batch == 0)
+    return finish(FERRAY_ERR_INVALID);
+  if (batch == 0 || m == 0 || n == 0) return finish(FERRAY_OK);
+  const size_t e = (dtype == FERRAY_DTYPE_F64 || dtype == FERRAY_DTYPE_C32) ? 8
+                   : dtype == FERRAY_DTYPE_F32                              ? 4
+                   : dtype == FERRAY_DTYPE_C64                              ? 16
+                                                                            : 1;
+  const size_t ce = dtype == FERRAY_DTYPE_I8 ? 4 : e;
+  const size_t na = std::min(a_batch, batch), nb = std::min(b_batch, batch);
+  return finish(staged(a, na * m * k * e, b, nb * k * n * e, c, batch * m * n * ce, false,
+                       [&](void* da, void* db, void* dc, hipStream_t st) {
+                         return ferray_matmul_batched_dev(dtype, batch, na, nb, m, k, n, da, db,
+                                                          dc, st);
                        }));
 }
 

@@ -184,6 +184,20 @@ int ferray_grouped_plan_launch(ferray_grouped_plan *plan,
 int ferray_grouped_plan_sync(ferray_grouped_plan *plan);
 int ferray_grouped_plan_destroy(ferray_grouped_plan *plan);
 
+/* Batched product with ferray's batch broadcasting: for i < batch,
+ *   C[i] = A[i % a_batch] @ B[i % b_batch]   (m x k @ k x n, row-major,
+ * matrices stored back to back), as in matmul_batched
+ * (products/mod.rs:579-659, indexing at :625-650), all in one grouped
+ * launch.  dtype: FERRAY_DTYPE_F64 / F32 / C64 / C32 / I8.  The host form
+ * follows the drop-in contract (synchronous, false on failure). */
+bool ferray_matmul_batched(int dtype, size_t batch, size_t a_batch,
+                           size_t b_batch, size_t m, size_t k, size_t n,
+                           const void *a, const void *b, void *c);
+int ferray_matmul_batched_dev(int dtype, size_t batch, size_t a_batch,
+                              size_t b_batch, size_t m, size_t k, size_t n,
+                              const void *a, const void *b, void *c,
+                              void *stream);
+
 /* Tile-shape table of the grouped kernel: entry [dtype*8 + b] is 0 (64x64
  * tiles) or 1 (128x128) for problems with ceil(log2(max(m, n_real))) =
  * clamp(., 4, 11) = b + 4; written by the package's autotuner. */
