@@ -17,6 +17,9 @@ Besides the headline the line carries:
   split_precision - the Ozaki-scheme DGEMM on INT8 MFMA at the same size, per
                   modulus count: time, speed-up over the native DGEMM above and
                   sampled normwise error, for normal(0,1) and u*2^e inputs;
+  split_sgemm   - SGEMM at the same size on BF16 matrix cores by operand
+                  splitting (3 and 6 piece products) against native FP32
+                  MFMA: time, speed-up and sampled normwise error;
   sweep         - the other dtypes of the GEMM family at their config sizes
                   (f32 8192, c64/c32 4096, i8 8192, i16 4096, bf16 8192),
                   one timed number each (--no-sweep skips them);
@@ -128,6 +131,41 @@ def split_precision(fa, dA, dB, n: int, steps: int, native_ms: float, dist: str,
         log(f"ozaki[{dist}] moduli={nm}: {ms:.3f} ms ({native_ms / ms:.2f}x native), "
             f"normwise {err_f / (na * nb):.2e}")
     dC.free()
+    return out
+
+
+def split_sgemm(fa, n: int, steps: int, warmup: int, samples: int = 16):
+    """SGEMM at n^3 (normal(0,1) f32): native FP32 MFMA against the bf16
+    operand-split forms (terms 3 and 6); time, speed-up and normwise error
+    of sampled rows against an f64 product of the same f32 operands."""
+    dA, dB, dC = fa.DeviceBuffer(n * n * 4), fa.DeviceBuffer(n * n * 4), fa.DeviceBuffer(n * n * 4)
+    fa.fill(dA, n * n, fa.FILL_F32_NORMAL, 21)
+    fa.fill(dB, n * n, fa.FILL_F32_NORMAL, 22)
+    L = fa.lib()
+    a = dA.download(np.float32, n * n).reshape(n, n)
+    b = dB.download(np.float32, n * n).reshape(n, n).astype(np.float64)
+    rows = np.sort(np.random.default_rng(5).choice(n, samples, replace=False))
+    ref = a[rows].astype(np.float64) @ b
+    scale = np.linalg.norm(a[rows].astype(np.float64)) * np.linalg.norm(b)
+    runs = {"native_f32": lambda: L.ferray_gemm_f32_dev(n, n, n, 1.0, dA.ptr, dB.ptr, 0.0, dC.ptr, None),
+            "bf16x3": lambda: L.ferray_sgemm_bf16x3_dev(n, n, n, 1.0, dA.ptr, dB.ptr, 0.0, dC.ptr, 3, None),
+            "bf16x6": lambda: L.ferray_sgemm_bf16x3_dev(n, n, n, 1.0, dA.ptr, dB.ptr, 0.0, dC.ptr, 6, None)}
+    out = {}
+    for name, fn in runs.items():
+        if fn() != 0:
+            out[name] = {"error": "launch failed"}
+            continue
+        ms = time_steps(fa, fn, max(3, steps // 2), max(1, warmup))
+        got = np.stack([dC.download(np.float32, n, offset_bytes=int(r) * n * 4) for r in rows])
+        err = float(np.linalg.norm(got.astype(np.float64) - ref) / scale)
+        out[name] = {"ms": round(ms, 4), "tflops_equiv": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12, 2),
+                     "normwise_ab_sampled_rows": err}
+        log(f"sgemm {name} {n}: {ms:.3f} ms, normwise {err:.2e}")
+    for name in ("bf16x3", "bf16x6"):
+        if "ms" in out.get(name, {}) and "ms" in out.get("native_f32", {}):
+            out[name]["speedup_vs_native"] = round(out["native_f32"]["ms"] / out[name]["ms"], 3)
+    for buf in (dA, dB, dC):
+        buf.free()
     return out
 
 
@@ -345,6 +383,8 @@ def main() -> int:
         buf.free()
     if not args.no_sweep:
         result["sweep"] = run_sweep(fa, args.steps, args.warmup)
+    if not args.no_split:
+        result["split_sgemm"] = split_sgemm(fa, n, args.steps, args.warmup)
     if not args.no_serve:
         result["serving"] = serving(fa)
         b = result["serving"]["best_at_p99_target"]["grouped"]

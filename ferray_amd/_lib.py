@@ -59,6 +59,8 @@ SIGNATURES: dict[str, tuple] = {
     "ferray_grouped_plan_destroy": (_i, [_p]),
     "ferray_set_tile_table": (_i, [_p, _sz]),
     "ferray_get_tile_table": (_i, [_p, _sz]),
+    "ferray_sgemm_bf16x3": (_b, [_sz, _sz, _sz, _f, _p, _p, _f, _p, _i]),
+    "ferray_sgemm_bf16x3_dev": (_i, [_sz, _sz, _sz, _f, _p, _p, _f, _p, _i, _p]),
     "ferray_matmul_batched": (_b, [_i, _sz, _sz, _sz, _sz, _sz, _sz, _p, _p, _p]),
     "ferray_matmul_batched_dev": (_i, [_i, _sz, _sz, _sz, _sz, _sz, _sz, _p, _p, _p, _p]),
     "ferray_server_create": (_i, [_p, C.POINTER(_p)]),

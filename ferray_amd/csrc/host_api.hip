@@ -102,6 +102,17 @@ bool ferray_matmul_batched(int dtype, size_t batch, size_t a_batch, size_t b_bat
                        }));
 }
 
+bool ferray_sgemm_bf16x3(size_t m, size_t n, size_t k, float alpha, const float* a,
+                         const float* b, float beta, float* c, int terms) {
+  if (m == 0 || n == 0) return finish(FERRAY_OK);
+  return finish(staged(a, m * k * 4, b, k * n * 4, c, m * n * 4, beta != 0.0f,
+                       [&](void* da, void* db, void* dc, hipStream_t st) {
+                         return ferray_sgemm_bf16x3_dev(m, n, k, alpha, (const float*)da,
+                                                        (const float*)db, beta, (float*)dc, terms,
+                                                        st);
+                       }));
+}
+
 bool ferray_dgemm_ozaki(size_t m, size_t n, size_t k, double alpha, const double* a,
                         const double* b, double beta, double* c, int moduli) {
   if (m == 0 || n == 0) return finish(FERRAY_OK);

@@ -22,7 +22,7 @@ __all__ = [
     "gpu_supports", "gemm_f64", "gemm_f32", "gemm_c64", "gemm_c32", "gemm_i8",
     "gemm_i8_signed", "gemm_i16", "quantized_matmul", "gemm_bf16_f32",
     "gemm_f16_f32", "dgemm_ozaki", "last_status", "DeviceBuffer", "fill", "sync",
-    "FILL_F64_UNIFORM", "FILL_F64_NORMAL", "FILL_F64_WIDE", "FILL_F32_UNIFORM",
+    "sgemm_bf16x3", "FILL_F64_UNIFORM", "FILL_F64_NORMAL", "FILL_F64_WIDE", "FILL_F32_UNIFORM",
     "FILL_F32_NORMAL", "FILL_I8", "FILL_I16", "FILL_U8", "Event", "device_sync",
 ]
 
@@ -78,6 +78,14 @@ def gemm_f32(m, n, k, alpha, a, b, beta, c) -> bool:
     _need(a, np.float32, m * k, "a"); _need(b, np.float32, k * n, "b")
     _need(c, np.float32, m * n, "c", True)
     return bool(lib().ferray_gemm_f32(m, n, k, alpha, _ptr(a), _ptr(b), beta, _ptr(c)))
+
+
+def sgemm_bf16x3(m, n, k, alpha, a, b, beta, c, terms: int = 3) -> bool:
+    """f32 C = alpha*A@B + beta*C on BF16 matrix cores by operand splitting
+    (terms 3: ~2^-16 relative error per product; terms 6: f32 level)."""
+    _need(a, np.float32, m * k, "a"); _need(b, np.float32, k * n, "b")
+    _need(c, np.float32, m * n, "c", True)
+    return bool(lib().ferray_sgemm_bf16x3(m, n, k, alpha, _ptr(a), _ptr(b), beta, _ptr(c), terms))
 
 
 def _cplx_view(arr: np.ndarray, real_dtype, count: int, name: str, writable=False) -> np.ndarray:

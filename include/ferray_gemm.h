@@ -140,6 +140,20 @@ int ferray_dgemm_ozaki_dev(size_t m, size_t n, size_t k, double alpha,
 /* Largest supported number of moduli. */
 int ferray_ozaki_max_moduli(void);
 
+/* C = alpha*A@B + beta*C in f32 computed on BF16 matrix cores: each f32
+ * operand is split into bf16 pieces (x = p0 + p1 + p2 exactly) and the piece
+ * products are summed in one f32-accumulating GEMM of depth terms*k.
+ * terms = 3 (0 = 3): p0p0 + p0p1 + p1p0, ~2^-16 relative error per
+ * product; terms = 6: adds p0p2 + p1p1 + p2p0, f32-level accuracy.
+ * No counterpart in the reference (its SGEMM is gemm_f32, mod.rs:940-1062);
+ * same argument order and meaning as ferray_gemm_f32 plus `terms`. */
+bool ferray_sgemm_bf16x3(size_t m, size_t n, size_t k, float alpha,
+                         const float *a, const float *b, float beta, float *c,
+                         int terms);
+int ferray_sgemm_bf16x3_dev(size_t m, size_t n, size_t k, float alpha,
+                            const float *a, const float *b, float beta,
+                            float *c, int terms, void *stream);
+
 /* ---------------- grouped (batched, mixed-dtype) GEMM ---------------- */
 
 /* Element types of a grouped problem. */
