@@ -22,6 +22,8 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <type_traits>
 
 #include "mfma_nt.hpp"
 
@@ -36,12 +38,19 @@ static const int kModuli[OZ_MAX_MODULI] = {256, 255, 253, 251, 247, 241, 239, 23
 struct OzConsts {
   int nmod;
   int m[OZ_MAX_MODULI];
-  float inv_m[OZ_MAX_MODULI];      // 1/m, for the residue reduction
-  unsigned c16[OZ_MAX_MODULI][3];  // 2^16, 2^32, 2^48 mod m
+  double md[OZ_MAX_MODULI];        // m, as a double
+  double inv_md[OZ_MAX_MODULI];    // 1/m, for the residue reduction
+  double c17[OZ_MAX_MODULI][2];    // 2^17 mod m, 2^34 mod m
   double w[OZ_MAX_MODULI][4];      // 32-bit pieces of the CRT weight W_l
   double mp[4];                    // 32-bit pieces of M
   double inv_M;                    // 1/M (approximate)
 };
+
+// Constants for every modulus count, indexed by the count (uploaded once per
+// process, before the first split-precision call).  Kernels read them from
+// the constant address space, so each one is fetched by a scalar load where
+// it is used instead of occupying registers for the whole kernel.
+__constant__ OzConsts g_oz[OZ_MAX_MODULI + 1];
 
 // ---------------------------------------------------------------------
 // Scaling
@@ -158,96 +167,122 @@ __global__ void oz_init(int* cbad, int N) {
 // modulo c.m[l], from 16-bit limbs of |x|: s = d0 + d1*(2^16 mod m) +
 // d2*(2^32 mod m) + d3*(2^48 mod m) < 2^26, reduced with a float reciprocal
 // whose quotient is off by at most one.
+// An integer-valued double x, |x| < 2^51, as x = l0 + l1 2^17 + l2 2^34 with
+// |l_i| < 2^17, every piece carrying the sign of x (all steps exact).
 struct Limbs {
-  unsigned d0, d1, d2, d3;
-  bool neg;
+  double l0, l1, l2;
 };
 __device__ __forceinline__ Limbs limbs_of(double x) {
-  Limbs L;
-  L.neg = x < 0.0;
-  const double ax = fabs(x);
-  const double hi = floor(ax * 0x1.0p-32);       // exact: < 2^31
-  const unsigned h = (unsigned)hi;
-  const unsigned l = (unsigned)fma(-hi, 0x1.0p32, ax);  // exact: < 2^32
-  L.d0 = l & 0xffff;
-  L.d1 = l >> 16;
-  L.d2 = h & 0xffff;
-  L.d3 = h >> 16;
-  return L;
+  const double l2 = trunc(x * 0x1.0p-34);
+  const double rem = fma(-l2, 0x1.0p34, x);
+  const double l1 = trunc(rem * 0x1.0p-17);
+  return Limbs{fma(-l1, 0x1.0p17, rem), l1, l2};
 }
+
+// Symmetric residue of x modulo m = c.m[l], in DP arithmetic that is exact
+// throughout: s = l0 + l1 (2^17 mod m) + l2 (2^34 mod m) is an integer with
+// |s| < 2^26 congruent to x; q = rint(s / m) through the double reciprocal is
+// the correctly rounded quotient (s / m is never within 1/512 of a
+// half-integer unless it is one, and the reciprocal's error is below 1e-13),
+// so r = s - q m lies in [-m/2, m/2]: [-(m-1)/2, (m-1)/2] for odd m, and
+// for m = 256 the ends -128 and 128 are the same byte.
 __device__ __forceinline__ int sym_residue(const Limbs& L, const OzConsts& c, int l) {
-  const int m = c.m[l];
-  const unsigned s = L.d0 + L.d1 * c.c16[l][0] + L.d2 * c.c16[l][1] + L.d3 * c.c16[l][2];
-  int q = (int)((float)s * c.inv_m[l]);
-  int r = (int)s - q * m;
-  if (r < 0) r += m;
-  if (r >= m) r -= m;
-  if (L.neg && r) r = m - r;
-  if (r > (m - 1) / 2) r -= m;
-  return r;
+  const double s = fma(L.l2, c.c17[l][1], fma(L.l1, c.c17[l][0], L.l0));
+  const double q = rint(s * c.inv_md[l]);
+  return (int)fma(-q, c.md[l], s);
 }
 
-// A planes: plane l is Mp x Kp int8, row-major.  Each thread covers 16
-// consecutive k of one row.
+// Residue bytes of 4 values modulo c.m[l], packed little-endian.
+__device__ __forceinline__ unsigned residue_word(const Limbs (&L)[4], const OzConsts& c, int l) {
+  unsigned v = 0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) v |= ((unsigned)sym_residue(L[e], c, l) & 0xffu) << (8 * e);
+  return v;
+}
+
+// A planes: plane l is Mp x Kp int8, row-major.  Each thread covers 4
+// consecutive k of one row, lanes running along k, so the f64 loads and the
+// 4-byte residue stores of a wave are both contiguous.  vec: rows are 16-byte
+// aligned (K even, A 16-byte aligned), so the 4 values load as 2 x 16 bytes.
+template <int NM>
 __global__ __launch_bounds__(256) void oz_residues_a(const double* __restrict__ a, int M, int K,
-                                                     int Mp, int Kp, const int* __restrict__ shift,
+                                                     int Mp, int Kp, int vec,
+                                                     const int* __restrict__ shift,
                                                      uint8_t* __restrict__ planes,
-                                                     size_t plane_bytes, OzConsts c) {
+                                                     size_t plane_bytes) {
+  const OzConsts& c = g_oz[NM];
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t per_row = Kp / 16;
+  const size_t per_row = Kp / 4;
   if (idx >= (size_t)Mp * per_row) return;
-  const int row = (int)(idx / per_row), k0 = (int)(idx % per_row) * 16;
-  Limbs L[16];
+  const int row = (int)(idx / per_row), k0 = (int)(idx % per_row) * 4;
+  double x[4] = {0.0, 0.0, 0.0, 0.0};
+  if (row < M) {
+    const double* src = a + (size_t)row * K + k0;
+    if (vec && k0 + 4 <= K) {
+      typedef double d2 __attribute__((ext_vector_type(2)));
+      const d2 lo = *reinterpret_cast<const d2*>(src);
+      const d2 hi = *reinterpret_cast<const d2*>(src + 2);
+      x[0] = lo.x;
+      x[1] = lo.y;
+      x[2] = hi.x;
+      x[3] = hi.y;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (k0 + e < K) x[e] = src[e];
+    }
+  }
   const int sh = row < M ? shift[row] : 0;
+  Limbs L[4];
 #pragma unroll
-  for (int e = 0; e < 16; ++e) {
-    const int k = k0 + e;
-    double x = (row < M && k < K) ? a[(size_t)row * K + k] : 0.0;
-    if (!isfinite(x)) x = 0.0;
-    L[e] = limbs_of(rint(ldexp(x, sh)));
-  }
+  for (int e = 0; e < 4; ++e) L[e] = limbs_of(rint(ldexp(isfinite(x[e]) ? x[e] : 0.0, sh)));
   uint8_t* dst = planes + (size_t)row * Kp + k0;
-  for (int l = 0; l < c.nmod; ++l) {
-    uint8_t v[16];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = (uint8_t)(int8_t)sym_residue(L[e], c, l);
-    *reinterpret_cast<u4v*>(dst + l * plane_bytes) = __builtin_bit_cast(u4v, v);
-  }
+  for (int l = 0; l < NM; ++l)
+    *reinterpret_cast<unsigned*>(dst + l * plane_bytes) = residue_word(L, c, l);
 }
 
-// B^T planes: plane l is Np x Kp int8 (row n holds column n of B).  Tile of
-// 64 (k) x 64 (n) through LDS; each thread covers 16 consecutive k of one n.
+// B^T planes: plane l is Np x Kp int8 (row n holds column n of B).  A block
+// takes 128 k x 32 n of B: coalesced row loads, stored transposed in LDS;
+// then each thread turns 4 consecutive k of one n into residues, so a wave's
+// 4-byte stores cover two full 128-byte rows of a plane.  Grid: Np/32 x Kp/128.
+template <int NM>
 __global__ __launch_bounds__(256) void oz_residues_bt(const double* __restrict__ b, int K, int N,
-                                                      int Kp, int Np, const int* __restrict__ shift,
+                                                      int Kp, const int* __restrict__ shift,
                                                       uint8_t* __restrict__ planes,
-                                                      size_t plane_bytes, OzConsts c) {
-  __shared__ double tile[64][65];
+                                                      size_t plane_bytes) {
+  const OzConsts& c = g_oz[NM];
+  __shared__ double tile[32][130];
   const int tid = threadIdx.x;
-  const int n0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
-  for (int idx = tid; idx < 64 * 64; idx += 256) {
-    const int r = idx >> 6, cc = idx & 63;
-    const int k = k0 + r, n = n0 + cc;
-    tile[r][cc] = (k < K && n < N) ? b[(size_t)k * N + n] : 0.0;
+  const int n0 = blockIdx.x * 32, k0 = blockIdx.y * 128;
+  {
+    const int nl = tid & 31, n = n0 + nl;
+    for (int kr = tid >> 5; kr < 128; kr += 8) {
+      const int k = k0 + kr;
+      tile[nl][kr] = (k < K && n < N) ? b[(size_t)k * N + n] : 0.0;
+    }
   }
   __syncthreads();
-  const int nl = tid >> 2, kc = (tid & 3) * 16;
-  const int n = n0 + nl, kbase = k0 + kc;
-  if (n >= Np || kbase >= Kp) return;
-  const int sh = n < N ? shift[n] : 0;
-  Limbs L[16];
+  const int tk = tid & 31, tn = tid >> 5;
+  // 4 k x 4 n per thread: all 16 limbs first, then one pass per modulus.
+  Limbs L[4][4];
 #pragma unroll
-  for (int e = 0; e < 16; ++e) {
-    double x = tile[kc + e][nl];
-    if (!isfinite(x)) x = 0.0;
-    L[e] = limbs_of(rint(ldexp(x, sh)));
-  }
-  uint8_t* dst = planes + (size_t)n * Kp + kbase;
-  for (int l = 0; l < c.nmod; ++l) {
-    uint8_t v[16];
+  for (int j = 0; j < 4; ++j) {
+    const int nl = tn + 8 * j, n = n0 + nl;
+    const int sh = n < N ? shift[n] : 0;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = (uint8_t)(int8_t)sym_residue(L[e], c, l);
-    *reinterpret_cast<u4v*>(dst + l * plane_bytes) = __builtin_bit_cast(u4v, v);
+    for (int e = 0; e < 4; ++e) {
+      const double x = tile[nl][4 * tk + e];
+      L[j][e] = limbs_of(rint(ldexp(isfinite(x) ? x : 0.0, sh)));
+    }
   }
+  uint8_t* dst = planes + (size_t)(n0 + tn) * Kp + k0 + 4 * tk;
+#pragma unroll
+  for (int l = 0; l < NM; ++l)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      *reinterpret_cast<unsigned*>(dst + (size_t)(8 * j) * Kp + l * plane_bytes) =
+          residue_word(L[j], c, l);
 }
 
 // GEMM epilogue: residue of the exact i32 product modulo m_plane, as a byte.
@@ -273,14 +308,11 @@ struct EpiResidue {
 // Reconstruction: C = alpha * 2^-(s_i+t_j) * CRT(r_1..r_N) + beta * C.  Each
 // thread finishes 8 consecutive columns of one row (residue planes have a
 // row pitch of ldr >= N, a multiple of 8, so one 8-byte load per plane).
-__device__ __forceinline__ double crt_value(const double (&r)[OZ_MAX_MODULI], const OzConsts& k) {
-  double S0 = 0, S1 = 0, S2 = 0, S3 = 0;
-  for (int l = 0; l < k.nmod; ++l) {
-    S0 = fma(r[l], k.w[l][0], S0);
-    S1 = fma(r[l], k.w[l][1], S1);
-    S2 = fma(r[l], k.w[l][2], S2);
-    S3 = fma(r[l], k.w[l][3], S3);
-  }
+// Finishes the CRT sum from its four 32-bit piece sums S_p = sum_l r_l w_lp
+// (each exact: < 16 * 2^8 * 2^32): removes the multiple of M, evaluates the
+// balanced 32-bit digits, returns the integer value as a double.
+__device__ __forceinline__ double crt_finish(double S0, double S1, double S2, double S3,
+                                             const OzConsts& k) {
   const double two32 = 4294967296.0;
   const double xa = ((S3 * two32 + S2) * two32 + S1) * two32 + S0;
   const double q = rint(xa * k.inv_M);
@@ -298,6 +330,16 @@ __device__ __forceinline__ double crt_value(const double (&r)[OZ_MAX_MODULI], co
   return ((D3 * two32 + D2) * two32 + D1) * two32 + D0;
 }
 
+// An integer 0 <= u < 2^32 as a double: its bits under the exponent of 2^52.
+__device__ __forceinline__ double u32_to_double(unsigned u) {
+  return __hiloint2double(0x43300000, (int)u) - 4503599627370496.0;
+}
+
+// Each wave finishes 512 consecutive columns of one row.  Lane L first
+// reconstructs columns 8L .. 8L+7 from one 8-byte load per residue plane;
+// the values are then exchanged through LDS so that every store (and every
+// read of C when beta != 0) covers 64 consecutive doubles.
+template <int NM>
 __global__ __launch_bounds__(256) void oz_reconstruct(const uint8_t* __restrict__ res,
                                                       size_t plane_bytes, int ldr, int M, int N,
                                                       const int* __restrict__ sA,
@@ -306,32 +348,63 @@ __global__ __launch_bounds__(256) void oz_reconstruct(const uint8_t* __restrict_
                                                       const int* __restrict__ badB, double alpha,
                                                       double beta, double* __restrict__ c,
                                                       OzConsts k) {
-  const int per_row = ldr / 8;
-  const size_t total = (size_t)M * per_row;
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(idx / per_row), j0 = (int)(idx % per_row) * 8;
-    if (j0 >= N) continue;
-    unsigned long long packed[OZ_MAX_MODULI];
+  __shared__ double xch[4][512];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int segs = (N + 511) / 512;
+  const size_t unit = (size_t)blockIdx.x * 4 + w;
+  const bool live = unit < (size_t)M * segs;
+  const int i = live ? (int)(unit / segs) : 0;
+  const int base = live ? (int)(unit % segs) * 512 : 0;
+  const int j0 = base + 8 * lane;
+  double v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = 0.0;
+  if (live && j0 < N) {
     const uint8_t* src = res + (size_t)i * ldr + j0;
-    for (int l = 0; l < k.nmod; ++l)
-      packed[l] = *reinterpret_cast<const unsigned long long*>(src + (size_t)l * plane_bytes);
+    // Piece sums of the 8 columns, accumulated modulus by modulus.
+    double S[8][4];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) S[e][0] = S[e][1] = S[e][2] = S[e][3] = 0.0;
+    // All residue words are requested at once (the weights are kernel
+    // arguments: measured faster than constant-memory or LDS copies here).
+    unsigned long long pk[NM];
+#pragma unroll
+    for (int l = 0; l < NM; ++l)
+      pk[l] = *reinterpret_cast<const unsigned long long*>(src + (size_t)l * plane_bytes);
+#pragma unroll
+    for (int l = 0; l < NM; ++l) {
+      const double w0 = k.w[l][0], w1 = k.w[l][1], w2 = k.w[l][2], w3 = k.w[l][3];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const double r = u32_to_double((unsigned)(pk[l] >> (8 * e)) & 0xffu);
+        S[e][0] = fma(r, w0, S[e][0]);
+        S[e][1] = fma(r, w1, S[e][1]);
+        S[e][2] = fma(r, w2, S[e][2]);
+        S[e][3] = fma(r, w3, S[e][3]);
+      }
+    }
     const int si = sA[i], bi = badA[i];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int j = j0 + e;
       if (j >= N) break;
-      double r[OZ_MAX_MODULI];
-#pragma unroll
-      for (int l = 0; l < OZ_MAX_MODULI; ++l)
-        r[l] = l < k.nmod ? (double)((packed[l] >> (8 * e)) & 0xff) : 0.0;
-      double v = ldexp(crt_value(r, k), -(si + tB[j]));
-      if (bi | badB[j]) v = __builtin_nan("");
-      v *= alpha;
-      double* dst = c + (size_t)i * N + j;
-      if (beta != 0.0) v = fma(beta, *dst, v);
-      *dst = v;
+      double x = ldexp(crt_finish(S[e][0], S[e][1], S[e][2], S[e][3], k), -(si + tB[j]));
+      if (bi | badB[j]) x = __builtin_nan("");
+      v[e] = x;
     }
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) xch[w][8 * lane + e] = v[e];
+  __syncthreads();
+  if (!live) return;
+  double* row = c + (size_t)i * N;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int j = base + lane + 64 * q;
+    if (j >= N) break;
+    double x = xch[w][lane + 64 * q] * alpha;
+    if (beta != 0.0) x = fma(beta, row[j], x);
+    row[j] = x;
   }
 }
 
@@ -353,11 +426,9 @@ static int make_consts(int nmod, OzConsts& c, int& gamma) {
   for (int l = 0; l < nmod; ++l) {
     const int m = kModuli[l];
     c.m[l] = m;
-    c.inv_m[l] = 1.0f / (float)m;
-    for (int t = 0; t < 3; ++t) {
-      u128 p = ((u128)1 << (16 * (t + 1))) % (u128)m;
-      c.c16[l][t] = (unsigned)p;
-    }
+    c.md[l] = (double)m;
+    c.inv_md[l] = 1.0 / (double)m;
+    for (int t = 0; t < 2; ++t) c.c17[l][t] = (double)(unsigned)(((u128)1 << (17 * (t + 1))) % (u128)m);
     const u128 Ml = M / (u128)m;
     const unsigned mm = (unsigned)(Ml % (u128)m);
     unsigned inv = 0;
@@ -377,6 +448,45 @@ static int make_consts(int nmod, OzConsts& c, int& gamma) {
 
 static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Calls f(std::integral_constant<int, nmod>{}) for nmod in [2, 16].
+template <class F>
+static void with_moduli(int nmod, F&& f) {
+  switch (nmod) {
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 9: f(std::integral_constant<int, 9>{}); break;
+    case 10: f(std::integral_constant<int, 10>{}); break;
+    case 11: f(std::integral_constant<int, 11>{}); break;
+    case 12: f(std::integral_constant<int, 12>{}); break;
+    case 13: f(std::integral_constant<int, 13>{}); break;
+    case 14: f(std::integral_constant<int, 14>{}); break;
+    case 15: f(std::integral_constant<int, 15>{}); break;
+    default: f(std::integral_constant<int, 16>{}); break;
+  }
+}
+
+// Uploads the constant tables (once per process).
+static int oz_tables() {
+  static std::once_flag once;
+  static int status = FERRAY_OK;
+  std::call_once(once, [] {
+    static OzConsts host[OZ_MAX_MODULI + 1];
+    std::memset(host, 0, sizeof(host));
+    for (int nm = 2; nm <= OZ_MAX_MODULI; ++nm) {
+      int g = 0;
+      if ((status = make_consts(nm, host[nm], g))) return;
+    }
+    const hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(g_oz), host, sizeof(host));
+    if (e != hipSuccess) status = map_hip_error(e);
+  });
+  return status;
+}
+
 template <class S>
 static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a,
                      const double* b, double beta, double* c, int nmod, hipStream_t st) {
@@ -384,7 +494,9 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
   int gamma = 0;
   int s = make_consts(nmod, cst, gamma);
   if (s) return s;
-  const size_t Kp = align_up(k, S::KB), Mp = align_up(m, S::BM), Np = align_up(n, S::BN);
+  if ((s = oz_tables())) return s;
+  // Kp: a whole number of GEMM stages and of the 128-k residue tiles.
+  const size_t Kp = align_up(k, 128), Mp = align_up(m, S::BM), Np = align_up(n, S::BN);
   const size_t ldr = align_up(n, 8);
   const size_t a_plane = Mp * Kp, b_plane = Np * Kp, r_plane = align_up(m * ldr, 256);
   const int kslice = 256;
@@ -418,11 +530,15 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
                      ps, tB);
   FG_CHECK(hipGetLastError());
 
-  const size_t athreads = Mp * (Kp / 16);
-  hipLaunchKernelGGL(oz_residues_a, dim3((unsigned)cdiv(athreads, 256)), dim3(256), 0, st, a,
-                     (int)m, (int)k, (int)Mp, (int)Kp, sA, ws + o_pa, a_plane, cst);
-  hipLaunchKernelGGL(oz_residues_bt, dim3((unsigned)(Np / 64), (unsigned)cdiv(Kp, 64)), dim3(256),
-                     0, st, b, (int)k, (int)n, (int)Kp, (int)Np, tB, ws + o_pb, b_plane, cst);
+  const size_t athreads = Mp * (Kp / 4);
+  const int vec_a = (k % 2 == 0) && ((uintptr_t)a % 16 == 0);
+  with_moduli(nmod, [&](auto nm) {
+    constexpr int NM = decltype(nm)::value;
+    hipLaunchKernelGGL((oz_residues_a<NM>), dim3((unsigned)cdiv(athreads, 256)), dim3(256), 0, st,
+                       a, (int)m, (int)k, (int)Mp, (int)Kp, vec_a, sA, ws + o_pa, a_plane);
+    hipLaunchKernelGGL((oz_residues_bt<NM>), dim3((unsigned)(Np / 32), (unsigned)(Kp / 128)),
+                       dim3(256), 0, st, b, (int)k, (int)n, (int)Kp, tB, ws + o_pb, b_plane);
+  });
   FG_CHECK(hipGetLastError());
 
   NtOperands p{};
@@ -447,10 +563,14 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
   }
   if ((s = launch_mfma_nt<ELEM_I8, S, false>(p, nmod, e, st))) return s;
 
-  size_t blocks = cdiv(m * (ldr / 8), 256);
-  if (blocks > 65536 * 4) blocks = 65536 * 4;
-  hipLaunchKernelGGL(oz_reconstruct, dim3((unsigned)blocks), dim3(256), 0, st, ws + o_r, r_plane,
-                     (int)ldr, (int)m, (int)n, sA, tB, badA, badB, alpha, beta, c, cst);
+  const size_t units = m * cdiv(n, 512);  // one wave per 512 columns of a row
+  if (cdiv(units, 4) > 0x7fffffff) return FERRAY_ERR_INVALID;
+  with_moduli(nmod, [&](auto nm) {
+    constexpr int NM = decltype(nm)::value;
+    hipLaunchKernelGGL((oz_reconstruct<NM>), dim3((unsigned)cdiv(units, 4)), dim3(256), 0, st,
+                       ws + o_r, r_plane, (int)ldr, (int)m, (int)n, sA, tB, badA, badB, alpha,
+                       beta, c, cst);
+  });
   FG_CHECK(hipGetLastError());
   return FERRAY_OK;
 }
