@@ -41,9 +41,10 @@ struct OzConsts {
   double md[OZ_MAX_MODULI];        // m, as a double
   double inv_md[OZ_MAX_MODULI];    // 1/m, for the residue reduction
   double c17[OZ_MAX_MODULI][2];    // 2^17 mod m, 2^34 mod m
-  double w[OZ_MAX_MODULI][4];      // 32-bit pieces of the CRT weight W_l
-  double mp[4];                    // 32-bit pieces of M
+  double w[OZ_MAX_MODULI][4];      // radix-2^R pieces of the CRT weight W_l
+  double mp[4];                    // radix-2^R pieces of M
   double inv_M;                    // 1/M (approximate)
+  double rad, irad;                // 2^R and 2^-R
 };
 
 // Constants for every modulus count, indexed by the count (uploaded once per
@@ -98,13 +99,26 @@ __global__ __launch_bounds__(256) void oz_row_scale(const double* __restrict__ a
   const int row = blockIdx.x;
   const double* r = a + (size_t)row * K;
   if (threadIdx.x == 0) anybad = 0;
-  Ssq acc{INT_MIN, 0.0};
+  // Four independent partial sums so four loads are in flight per thread.
+  Ssq acc4[4] = {{INT_MIN, 0.0}, {INT_MIN, 0.0}, {INT_MIN, 0.0}, {INT_MIN, 0.0}};
   int nonfinite = 0;
-  for (int k = threadIdx.x; k < K; k += 256) {
-    const double x = r[k];
-    if (!isfinite(x)) nonfinite = 1;
-    acc = ssq_add(acc, x);
+  int k = threadIdx.x;
+  for (; k + 3 * 256 < K; k += 4 * 256) {
+    double x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = r[k + u * 256];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      nonfinite |= !isfinite(x[u]);
+      acc4[u] = ssq_add(acc4[u], x[u]);
+    }
   }
+  for (; k < K; k += 256) {
+    const double x = r[k];
+    nonfinite |= !isfinite(x);
+    acc4[0] = ssq_add(acc4[0], x);
+  }
+  const Ssq acc = ssq_merge(ssq_merge(acc4[0], acc4[1]), ssq_merge(acc4[2], acc4[3]));
   pe[threadIdx.x] = acc.E;
   ps[threadIdx.x] = acc.S;
   __syncthreads();
@@ -132,13 +146,25 @@ __global__ void oz_col_partial(const double* __restrict__ b, int K, int N, int k
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
   const int k0 = blockIdx.y * kslice, k1 = min(K, k0 + kslice);
-  Ssq acc{INT_MIN, 0.0};
+  Ssq acc4[4] = {{INT_MIN, 0.0}, {INT_MIN, 0.0}, {INT_MIN, 0.0}, {INT_MIN, 0.0}};
   int nonfinite = 0;
-  for (int k = k0; k < k1; ++k) {
-    const double x = b[(size_t)k * N + j];
-    if (!isfinite(x)) nonfinite = 1;
-    acc = ssq_add(acc, x);
+  int k = k0;
+  for (; k + 3 < k1; k += 4) {
+    double x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = b[(size_t)(k + u) * N + j];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      nonfinite |= !isfinite(x[u]);
+      acc4[u] = ssq_add(acc4[u], x[u]);
+    }
   }
+  for (; k < k1; ++k) {
+    const double x = b[(size_t)k * N + j];
+    nonfinite |= !isfinite(x);
+    acc4[0] = ssq_add(acc4[0], x);
+  }
+  const Ssq acc = ssq_merge(ssq_merge(acc4[0], acc4[1]), ssq_merge(acc4[2], acc4[3]));
   pe[(size_t)blockIdx.y * N + j] = acc.E;
   ps[(size_t)blockIdx.y * N + j] = acc.S;
   if (nonfinite) atomicOr(bad + j, 1);
@@ -308,26 +334,37 @@ struct EpiResidue {
 // Reconstruction: C = alpha * 2^-(s_i+t_j) * CRT(r_1..r_N) + beta * C.  Each
 // thread finishes 8 consecutive columns of one row (residue planes have a
 // row pitch of ldr >= N, a multiple of 8, so one 8-byte load per plane).
-// Finishes the CRT sum from its four 32-bit piece sums S_p = sum_l r_l w_lp
-// (each exact: < 16 * 2^8 * 2^32): removes the multiple of M, evaluates the
-// balanced 32-bit digits, returns the integer value as a double.
-__device__ __forceinline__ double crt_finish(double S0, double S1, double S2, double S3,
-                                             const OzConsts& k) {
-  const double two32 = 4294967296.0;
-  const double xa = ((S3 * two32 + S2) * two32 + S1) * two32 + S0;
+// Number of radix pieces the CRT weights are split into: 3 while
+// NM * 255 * 2^R stays below 2^52 with 3R >= log2(M) (up to 15 moduli), else
+// 4 pieces of 32 bits.
+template <int NM>
+struct CrtPieces {
+  static constexpr int value = NM <= 15 ? 3 : 4;
+};
+
+// Finishes the CRT sum from its piece sums S_p = sum_l r_l w_lp (each an
+// exact integer below 2^52): removes the multiple of M (exactly, piece by
+// piece), normalises the pieces to balanced radix-2^R digits and returns the
+// integer value as a double.
+template <int P>
+__device__ __forceinline__ double crt_finish(const double (&S)[P], const OzConsts& k) {
+  double xa = S[P - 1];
+#pragma unroll
+  for (int p = P - 2; p >= 0; --p) xa = xa * k.rad + S[p];
   const double q = rint(xa * k.inv_M);
-  double D0 = fma(-q, k.mp[0], S0), D1 = fma(-q, k.mp[1], S1);
-  double D2 = fma(-q, k.mp[2], S2), D3 = fma(-q, k.mp[3], S3);
-  double t = rint(D0 * (1.0 / two32));
-  D0 = fma(-t, two32, D0);
-  D1 += t;
-  t = rint(D1 * (1.0 / two32));
-  D1 = fma(-t, two32, D1);
-  D2 += t;
-  t = rint(D2 * (1.0 / two32));
-  D2 = fma(-t, two32, D2);
-  D3 += t;
-  return ((D3 * two32 + D2) * two32 + D1) * two32 + D0;
+  double D[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) D[p] = fma(-q, k.mp[p], S[p]);
+#pragma unroll
+  for (int p = 0; p + 1 < P; ++p) {
+    const double t = rint(D[p] * k.irad);
+    D[p] = fma(-t, k.rad, D[p]);
+    D[p + 1] += t;
+  }
+  double v = D[P - 1];
+#pragma unroll
+  for (int p = P - 2; p >= 0; --p) v = v * k.rad + D[p];
+  return v;
 }
 
 // An integer 0 <= u < 2^32 as a double: its bits under the exponent of 2^52.
@@ -362,9 +399,12 @@ __global__ __launch_bounds__(256) void oz_reconstruct(const uint8_t* __restrict_
   if (live && j0 < N) {
     const uint8_t* src = res + (size_t)i * ldr + j0;
     // Piece sums of the 8 columns, accumulated modulus by modulus.
-    double S[8][4];
+    constexpr int P = CrtPieces<NM>::value;
+    double S[8][P];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) S[e][0] = S[e][1] = S[e][2] = S[e][3] = 0.0;
+    for (int e = 0; e < 8; ++e)
+#pragma unroll
+      for (int q = 0; q < P; ++q) S[e][q] = 0.0;
     // All residue words are requested at once (the weights are kernel
     // arguments: measured faster than constant-memory or LDS copies here).
     unsigned long long pk[NM];
@@ -373,14 +413,11 @@ __global__ __launch_bounds__(256) void oz_reconstruct(const uint8_t* __restrict_
       pk[l] = *reinterpret_cast<const unsigned long long*>(src + (size_t)l * plane_bytes);
 #pragma unroll
     for (int l = 0; l < NM; ++l) {
-      const double w0 = k.w[l][0], w1 = k.w[l][1], w2 = k.w[l][2], w3 = k.w[l][3];
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const double r = u32_to_double((unsigned)(pk[l] >> (8 * e)) & 0xffu);
-        S[e][0] = fma(r, w0, S[e][0]);
-        S[e][1] = fma(r, w1, S[e][1]);
-        S[e][2] = fma(r, w2, S[e][2]);
-        S[e][3] = fma(r, w3, S[e][3]);
+#pragma unroll
+        for (int q = 0; q < P; ++q) S[e][q] = fma(r, k.w[l][q], S[e][q]);
       }
     }
     const int si = sA[i], bi = badA[i];
@@ -388,7 +425,7 @@ __global__ __launch_bounds__(256) void oz_reconstruct(const uint8_t* __restrict_
     for (int e = 0; e < 8; ++e) {
       const int j = j0 + e;
       if (j >= N) break;
-      double x = ldexp(crt_finish(S[e][0], S[e][1], S[e][2], S[e][3], k), -(si + tB[j]));
+      double x = ldexp(crt_finish<P>(S[e], k), -(si + tB[j]));
       if (bi | badB[j]) x = __builtin_nan("");
       v[e] = x;
     }
@@ -423,6 +460,13 @@ static int make_consts(int nmod, OzConsts& c, int& gamma) {
   int bits = 0;
   for (u128 t = M; t > 1; t >>= 1) ++bits;  // floor(log2 M)
   gamma = (bits - 2) / 2;
+  // CRT radix (see CrtPieces): 3 pieces of R = ceil((bits + 1) / 3) bits up to
+  // 15 moduli (R <= 40, so 15 * 255 * 2^R < 2^52), else 4 pieces of 32 bits.
+  const int pieces = nmod <= 15 ? 3 : 4;
+  const int R = pieces == 3 ? (bits + 1 + 2) / 3 : 32;
+  if (pieces * R < bits + 1 || (double)nmod * 255.0 * std::ldexp(1.0, R) >= std::ldexp(1.0, 52))
+    return FERRAY_ERR_INVALID;
+  const u128 mask = ((u128)1 << R) - 1;
   for (int l = 0; l < nmod; ++l) {
     const int m = kModuli[l];
     c.m[l] = m;
@@ -439,10 +483,13 @@ static int make_consts(int nmod, OzConsts& c, int& gamma) {
       }
     if (!inv) return FERRAY_ERR_INVALID;
     const u128 W = Ml * (u128)inv;  // < M since inv < m
-    for (int p = 0; p < 4; ++p) c.w[l][p] = (double)(unsigned)((W >> (32 * p)) & 0xffffffffu);
+    for (int p = 0; p < pieces; ++p)
+      c.w[l][p] = (double)(unsigned long long)((W >> (R * p)) & mask);
   }
-  for (int p = 0; p < 4; ++p) c.mp[p] = (double)(unsigned)((M >> (32 * p)) & 0xffffffffu);
+  for (int p = 0; p < pieces; ++p) c.mp[p] = (double)(unsigned long long)((M >> (R * p)) & mask);
   c.inv_M = 1.0 / (double)M;
+  c.rad = std::ldexp(1.0, R);
+  c.irad = std::ldexp(1.0, -R);
   return FERRAY_OK;
 }
 
