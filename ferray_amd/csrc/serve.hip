@@ -343,12 +343,7 @@ static bool launch(ferray_server* s, std::unique_lock<std::mutex>& lk) {
       p.beta_re = 0.0;
       p.beta_im = 0.0;
     }
-    // A batch of one goes straight to its single-problem entry point (same
-    // result bits, no descriptor upload or graph update).
-    if (!st && probs.size() == 1)
-      st = launch_single(snaps[0].r, probs[0].a, probs[0].b, probs[0].c, lane->stream);
-    else if (!st)
-      st = ferray_grouped_plan_launch(lane->plan, probs.data(), probs.size(), lane->stream);
+    if (!st) st = ferray_grouped_plan_launch(lane->plan, probs.data(), probs.size(), lane->stream);
     if (!st && hipEventRecord(lane->ev, lane->stream) != hipSuccess) st = FERRAY_ERR_HIP;
     lk.lock();
     ++s->stats.launches;
