@@ -65,7 +65,12 @@ void gemm_real_kernel(RealArgs<T> p) {
   constexpr int CV = BK * RB_N / (2 * V) / R_THREADS;  // B vectors per thread (complex)
   static_assert(AV >= 1 && BV >= 1 && CV >= 1, "loader shape");
   typedef T vec_t __attribute__((ext_vector_type(V)));
-  __shared__ T As[2][BK][RB_M];
+  // A rows are padded so that the transposed stores of one LDS phase (f64:
+  // 16 lanes = 4 rows x 4 k pairs; f32: 32 lanes = 8 rows x 4 k quads) fall
+  // in distinct banks; the f32 pad also keeps the two k rows that one
+  // fragment read touches on different banks.
+  constexpr int APAD = sizeof(T) == 8 ? 2 : 18;
+  __shared__ T As[2][BK][RB_M + APAD];
   __shared__ T Bs[2][BK][RB_N];
 
   const int M = p.M, N = p.N, K = p.K;

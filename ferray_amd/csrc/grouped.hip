@@ -199,7 +199,7 @@ __device__ __noinline__ void real_dispatch(const GProblem& P, int m0, int n0, ui
   r.alpha_one = (r.alpha_re == T(1) && r.alpha_im == T(0));
   r.beta_zero = (r.beta_re == T(0) && r.beta_im == T(0));
   T* As = reinterpret_cast<T*>(smem);
-  T* Bs = As + 2 * BK * BM;
+  T* Bs = As + 2 * BK * RealTileShape<T, BM, BN, BK>::A_PITCH;
   if (real_tile_interior<T, BM, BN, BK>(r, m0, n0))
     real_tile<T, CPLX, false, BM, BN, BK>(r, m0, n0, As, Bs);
   else

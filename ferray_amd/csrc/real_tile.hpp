@@ -62,14 +62,19 @@ template <typename T, int BM, int BN, int BK>
 struct RealTileShape {
   static constexpr int V = 16 / sizeof(T);  // elements per 16-byte vector
   static constexpr int A_VECS = BM * BK / V;
+  // Row pitch of the transposed A tile: padded so that the transposed stores
+  // of one LDS phase fall in distinct banks (f64: 16 lanes = 4 rows x 4 k
+  // pairs; f32: 32 lanes = 8 rows x 4 k quads), and the two k rows one f32
+  // fragment read touches sit on different banks.
+  static constexpr int A_PITCH = BM + (sizeof(T) == 8 ? 2 : 18);
   static constexpr int B_VECS = BK * BN / V;          // real B
   static constexpr int C_VECS = BK * BN / (2 * V);    // complex B (pairs expand to 2x2)
-  static constexpr size_t lds_bytes = (size_t)2 * BK * (BM + BN) * sizeof(T);
+  static constexpr size_t lds_bytes = (size_t)2 * BK * (A_PITCH + BN) * sizeof(T);
 };
 
 // Computes the BM x BN tile at (m0, n0).  EDGE = false requires the whole
 // tile and every k stage in range and 16-byte aligned rows; EDGE = true
-// guards every element.  As holds 2*BK*BM elements, Bs 2*BK*BN (LDS).
+// guards every element.  As holds 2*BK*A_PITCH elements, Bs 2*BK*BN (LDS).
 template <typename T, bool CPLX, bool EDGE, int BM, int BN, int BK>
 __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, T* As, T* Bs) {
   using MF = Mfma16x16x4<T>;
@@ -82,7 +87,8 @@ __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, 
   constexpr int BV = (BVN + RT_THREADS - 1) / RT_THREADS;
   static_assert(BM % 32 == 0 && BN % 32 == 0 && BK % 4 == 0 && BK % V == 0, "tile shape");
   typedef T vec_t __attribute__((ext_vector_type(V)));
-  // As: [2][BK][BM], Bs: [2][BK][BN]
+  // As: [2][BK][A_PITCH], Bs: [2][BK][BN]
+  constexpr int AP = S::A_PITCH;
 
   const int M = p.M, N = p.N, K = p.K;
   const int tid = threadIdx.x;
@@ -147,7 +153,7 @@ __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, 
     }
   };
   auto store = [&](int buf) {
-    T* as = As + buf * BK * BM;
+    T* as = As + buf * BK * AP;
     T* bs = Bs + buf * BK * BN;
 #pragma unroll
     for (int q = 0; q < AV; ++q) {
@@ -155,7 +161,7 @@ __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, 
       if (AV * RT_THREADS > S::A_VECS && v >= S::A_VECS) break;
       const int row = v / (BK / V), kc = (v % (BK / V)) * V;
 #pragma unroll
-      for (int e = 0; e < V; ++e) as[(kc + e) * BM + row] = ra[q][e];
+      for (int e = 0; e < V; ++e) as[(kc + e) * AP + row] = ra[q][e];
     }
 #pragma unroll
     for (int q = 0; q < BV; ++q) {
@@ -193,13 +199,13 @@ __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, 
     const int cur = kt & 1;
     const bool more = kt + 1 < nk;
     if (more) load((kt + 1) * BK);
-    const T* as = As + cur * BK * BM;
+    const T* as = As + cur * BK * AP;
     const T* bs = Bs + cur * BK * BN;
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
       T af[TI], bf[TJ];
 #pragma unroll
-      for (int i = 0; i < TI; ++i) af[i] = as[(kk * 4 + lk) * BM + wm0 + i * 16 + lr];
+      for (int i = 0; i < TI; ++i) af[i] = as[(kk * 4 + lk) * AP + wm0 + i * 16 + lr];
 #pragma unroll
       for (int j = 0; j < TJ; ++j) bf[j] = bs[(kk * 4 + lk) * BN + wn0 + j * 16 + lr];
 #pragma unroll
