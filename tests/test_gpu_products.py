@@ -111,3 +111,71 @@ def test_batched_int8_through_grouped(gpu):
     assert ok
     ref = np.matmul(a.astype(np.int64), b.astype(np.int64))
     assert np.array_equal(out, ref.astype(np.int32))
+
+
+@pytest.mark.parametrize("kind", ["i8", "u8", "i16"])
+def test_matmul_int_exact(gpu, kind):
+    r = rng(8)
+    m, k, n = 37, 300, 45
+    if kind == "i8":
+        a = r.integers(-128, 128, (m, k)).astype(np.int8)
+        b = r.integers(-128, 128, (k, n)).astype(np.int8)
+    elif kind == "u8":
+        a = r.integers(0, 256, (m, k)).astype(np.uint8)
+        b = r.integers(-128, 128, (k, n)).astype(np.int8)
+    else:
+        a = r.integers(-32768, 32768, (m, k)).astype(np.int16)
+        b = r.integers(-32768, 32768, (k, n)).astype(np.int16)
+    got = pr.matmul_int(a, b)
+    ref = (a.astype(np.int64) @ b.astype(np.int64)) & 0xFFFFFFFF
+    assert got.dtype == np.int32
+    assert np.array_equal(got.astype(np.int64) & 0xFFFFFFFF, ref)
+
+
+def test_matrix_power_fibonacci_exact(gpu):
+    # [[1,1],[1,0]]^n = [[F(n+1), F(n)], [F(n), F(n-1)]], exact in f64 for n <= 70
+    q = np.array([[1.0, 1.0], [1.0, 0.0]])
+    fib = [0, 1]
+    for _ in range(80):
+        fib.append(fib[-1] + fib[-2])
+    for n in (1, 2, 5, 31, 70):
+        got = pr.matrix_power(q, n)
+        assert got.tolist() == [[fib[n + 1], fib[n]], [fib[n], fib[n - 1]]]
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128, np.float32])
+def test_matrix_power_against_numpy(gpu, dt):
+    r = rng(9)
+    sz = 96
+    a = (np.eye(sz) * 0.5 + 0.02 * r.standard_normal((sz, sz))).astype(dt)
+    if np.iscomplexobj(a):
+        a = a + 0.02j * r.standard_normal((sz, sz))
+    for p in (0, 1, 3, 12):
+        got = pr.matrix_power(a, p)
+        ref = np.linalg.matrix_power(a.astype(np.complex128 if np.iscomplexobj(a) else np.float64), p)
+        tol = 1e-12 if dt != np.float32 else 2e-5
+        assert np.linalg.norm(got - ref) <= tol * max(1.0, np.linalg.norm(ref))
+
+
+def test_multi_dot_against_numpy(gpu):
+    r = rng(10)
+    shapes = [(10, 300), (300, 5), (5, 200), (200, 40)]
+    mats = [r.standard_normal(s) for s in shapes]
+    got = pr.multi_dot(mats)
+    ref = np.linalg.multi_dot(mats)
+    assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+    v0, v1 = r.standard_normal(10), r.standard_normal(40)
+    got = pr.multi_dot([v0] + mats + [v1])
+    ref = np.linalg.multi_dot([v0] + mats + [v1])
+    assert abs(float(got) - float(ref)) <= 1e-10 * max(1.0, abs(float(ref)))
+
+
+def test_einsum_matmul(gpu):
+    r = rng(11)
+    a, b = r.standard_normal((20, 30)), r.standard_normal((30, 25))
+    for s in ("ij,jk->ik", "ab,bc->ac"):
+        got = pr.einsum_matmul(s, a, b)
+        assert np.linalg.norm(got - np.einsum(s, a, b)) <= 1e-13 * np.linalg.norm(a @ b)
+    x, y = r.standard_normal((4, 20, 30)), r.standard_normal((4, 30, 25))
+    got = pr.einsum_matmul("bij,bjk->bik", x, y)
+    assert np.linalg.norm(got - np.einsum("bij,bjk->bik", x, y)) <= 1e-13 * np.linalg.norm(got)

@@ -57,3 +57,27 @@ def test_empty_products_need_no_device():
     assert pr.gemm(z(0, 3), z(3, 4)).shape == (0, 4)
     assert pr.gemm(z(2, 0), z(0, 4)).tolist() == [[0.0] * 4] * 2
     assert pr.matmul_batched(z(0, 2, 3), z(3, 4)).shape == (0, 2, 4)
+
+
+def test_consumer_argument_errors():
+    with pytest.raises(pr.ShapeMismatch):
+        pr.matrix_power(z(2, 3), 2)
+    with pytest.raises(ValueError):
+        pr.matrix_power(z(2, 2), -1)
+    assert pr.matrix_power(z(3, 3), 0).tolist() == np.eye(3).tolist()
+    with pytest.raises(ValueError):
+        pr.multi_dot([z(2, 2)])
+    with pytest.raises(pr.ShapeMismatch):
+        pr.multi_dot([z(2, 3), z(4, 5), z(5, 2)])
+    with pytest.raises(pr.ShapeMismatch):
+        pr.multi_dot([z(2, 3), z(3, 4, 1), z(4, 2)])
+    with pytest.raises(NotImplementedError):
+        pr.einsum_matmul("ij,jk", z(2, 2), z(2, 2))
+    with pytest.raises(NotImplementedError):
+        pr.einsum_matmul("ij,kj->ik", z(2, 2), z(2, 2))
+    with pytest.raises(NotImplementedError):
+        pr.einsum_matmul("ii,ij->ij", z(2, 2), z(2, 2))
+    with pytest.raises(TypeError):
+        pr.matmul_int(z(2, 2, dt=np.int8), z(2, 2, dt=np.int16))
+    with pytest.raises(pr.ShapeMismatch):
+        pr.matmul_int(z(2, 3, dt=np.int8), z(2, 2, dt=np.int8))
