@@ -59,6 +59,11 @@ SIGNATURES: dict[str, tuple] = {
     "ferray_grouped_plan_destroy": (_i, [_p]),
     "ferray_set_tile_table": (_i, [_p, _sz]),
     "ferray_get_tile_table": (_i, [_p, _sz]),
+    "ferray_trsm_left_lower_f64": (_b, [_sz, _sz, _d, _p, _sz, _p, _sz, _b]),
+    "ferray_trsm_left_upper_f64": (_b, [_sz, _sz, _d, _p, _sz, _p, _sz, _b]),
+    "ferray_trsm_left_lower_f32": (_b, [_sz, _sz, _f, _p, _sz, _p, _sz, _b]),
+    "ferray_trsm_left_upper_f32": (_b, [_sz, _sz, _f, _p, _sz, _p, _sz, _b]),
+    "ferray_trsm_dev": (_i, [_i, _i, _sz, _sz, _d, _p, _sz, _p, _sz, _b, _p]),
     "ferray_sgemm_bf16x3": (_b, [_sz, _sz, _sz, _f, _p, _p, _f, _p, _i]),
     "ferray_sgemm_bf16x3_dev": (_i, [_sz, _sz, _sz, _f, _p, _p, _f, _p, _i, _p]),
     "ferray_matmul_batched": (_b, [_i, _sz, _sz, _sz, _sz, _sz, _sz, _p, _p, _p]),

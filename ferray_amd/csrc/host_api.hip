@@ -102,6 +102,46 @@ bool ferray_matmul_batched(int dtype, size_t batch, size_t a_batch, size_t b_bat
                        }));
 }
 
+bool ferray_trsm_left_lower_f64(size_t m, size_t n, double alpha, const double* a, size_t lda,
+                                double* b, size_t ldb, bool unit_diag) {
+  if (m == 0 || n == 0) return finish(FERRAY_OK);
+  return finish(staged(a, m * lda * 8, nullptr, 0, b, m * ldb * 8, true,
+                       [&](void* da, void*, void* db, hipStream_t st) {
+                         return ferray_trsm_dev(FERRAY_DTYPE_F64, 0, m, n, alpha, da, lda, db, ldb,
+                                                unit_diag, st);
+                       }));
+}
+
+bool ferray_trsm_left_upper_f64(size_t m, size_t n, double alpha, const double* a, size_t lda,
+                                double* b, size_t ldb, bool unit_diag) {
+  if (m == 0 || n == 0) return finish(FERRAY_OK);
+  return finish(staged(a, m * lda * 8, nullptr, 0, b, m * ldb * 8, true,
+                       [&](void* da, void*, void* db, hipStream_t st) {
+                         return ferray_trsm_dev(FERRAY_DTYPE_F64, 1, m, n, alpha, da, lda, db, ldb,
+                                                unit_diag, st);
+                       }));
+}
+
+bool ferray_trsm_left_lower_f32(size_t m, size_t n, float alpha, const float* a, size_t lda,
+                                float* b, size_t ldb, bool unit_diag) {
+  if (m == 0 || n == 0) return finish(FERRAY_OK);
+  return finish(staged(a, m * lda * 4, nullptr, 0, b, m * ldb * 4, true,
+                       [&](void* da, void*, void* db, hipStream_t st) {
+                         return ferray_trsm_dev(FERRAY_DTYPE_F32, 0, m, n, alpha, da, lda, db, ldb,
+                                                unit_diag, st);
+                       }));
+}
+
+bool ferray_trsm_left_upper_f32(size_t m, size_t n, float alpha, const float* a, size_t lda,
+                                float* b, size_t ldb, bool unit_diag) {
+  if (m == 0 || n == 0) return finish(FERRAY_OK);
+  return finish(staged(a, m * lda * 4, nullptr, 0, b, m * ldb * 4, true,
+                       [&](void* da, void*, void* db, hipStream_t st) {
+                         return ferray_trsm_dev(FERRAY_DTYPE_F32, 1, m, n, alpha, da, lda, db, ldb,
+                                                unit_diag, st);
+                       }));
+}
+
 bool ferray_sgemm_bf16x3(size_t m, size_t n, size_t k, float alpha, const float* a,
                          const float* b, float beta, float* c, int terms) {
   if (m == 0 || n == 0) return finish(FERRAY_OK);

@@ -1,0 +1,216 @@
+// Triangular solve op(A) X = alpha B (left side, no transpose, lower or upper,
+// row-major with row strides lda, ldb; X overwrites B), the GEMM consumer of
+// ferray-linalg/src/trsm.rs.
+//
+// Same block recursion as the reference (trsm.rs:165-200, :264-301): split
+// the rows at m1 = min(round_up(m/2, NB), m - NB), solve one half, update the
+// other with a GEMM, solve it.  Here the trailing update runs on the MFMA
+// DGEMM/SGEMM (C = -1 * A21 X1 + 1 * B2; strided blocks are first copied to
+// tight workspace with 2-D copies), and the leaf (m <= NB = 64 rows) is a
+// kernel in which each thread solves one column of B by substitution with the
+// triangular block in LDS.  The leaf applies the reference's scalar
+// operation order (row i: subtract a_ip x_p for p in order, then multiply by
+// 1 / a_ii), so a solve that fits one leaf gives the reference's bits.
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace fg {
+
+constexpr int TRSM_NB = 64;
+
+template <typename T>
+__global__ void trsm_scale_kernel(int m, int n, T alpha, T* __restrict__ b, size_t ldb) {
+  const size_t total = (size_t)m * n;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = i / n, c = i % n;
+    b[r * ldb + c] *= alpha;
+  }
+}
+
+// Leaf solve of an m x m (m <= NB) triangle; thread j owns column j of B.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void trsm_leaf_kernel(int m, int n, const T* __restrict__ a,
+                                                        size_t lda, T* __restrict__ b, size_t ldb,
+                                                        int unit_diag) {
+  constexpr int NB = TRSM_NB;
+  __shared__ T As[NB][NB + 1];
+  __shared__ T inv_diag[NB];
+  for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+    const int r = idx / NB, c = idx % NB;
+    As[r][c] = (r < m && c < m) ? a[(size_t)r * lda + c] : T(0);
+  }
+  __syncthreads();
+  if (threadIdx.x < NB) inv_diag[threadIdx.x] = threadIdx.x < m ? T(1) / As[threadIdx.x][threadIdx.x] : T(0);
+  __syncthreads();
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  T x[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) x[i] = i < m ? b[(size_t)i * ldb + j] : T(0);
+  if (LOWER) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      if (i < m) {
+        T v = x[i];
+#pragma unroll
+        for (int p = 0; p < i; ++p) v -= As[i][p] * x[p];
+        if (!unit_diag) v *= inv_diag[i];
+        x[i] = v;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = NB - 1; i >= 0; --i) {
+      if (i < m) {
+        T v = x[i];
+#pragma unroll
+        for (int p = i + 1; p < NB; ++p)
+          if (p < m) v -= As[i][p] * x[p];
+        if (!unit_diag) v *= inv_diag[i];
+        x[i] = v;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+    if (i < m) b[(size_t)i * ldb + j] = x[i];
+}
+
+template <typename T>
+int gemm_dev(size_t m, size_t n, size_t k, T alpha, const T* a, const T* b, T beta, T* c,
+             hipStream_t st);
+template <>
+int gemm_dev<double>(size_t m, size_t n, size_t k, double alpha, const double* a, const double* b,
+                     double beta, double* c, hipStream_t st) {
+  return ferray_gemm_f64_dev(m, n, k, alpha, a, b, beta, c, st);
+}
+template <>
+int gemm_dev<float>(size_t m, size_t n, size_t k, float alpha, const float* a, const float* b,
+                    float beta, float* c, hipStream_t st) {
+  return ferray_gemm_f32_dev(m, n, k, alpha, a, b, beta, c, st);
+}
+
+template <typename T>
+struct TrsmCtx {
+  int n;
+  size_t lda, ldb;
+  bool upper, unit;
+  T* ws_a;  // tight copy of the off-diagonal block
+  T* ws_x;  // tight copies of the solved and the updated rows (ldb != n)
+  T* ws_b;
+  hipStream_t st;
+};
+
+static size_t trsm_split(size_t m) {
+  const size_t half = (m / 2 + TRSM_NB - 1) / TRSM_NB * TRSM_NB;
+  return std::min(half, m - TRSM_NB);
+}
+
+// dst (rows x cols, tight) <- src (row stride ld), or the reverse.
+template <typename T>
+static int copy2d(T* dst, size_t dld, const T* src, size_t sld, size_t rows, size_t cols,
+                  hipStream_t st) {
+  if (!rows || !cols) return FERRAY_OK;
+  FG_CHECK(hipMemcpy2DAsync(dst, dld * sizeof(T), src, sld * sizeof(T), cols * sizeof(T), rows,
+                            hipMemcpyDeviceToDevice, st));
+  return FERRAY_OK;
+}
+
+// B_t -= A_ts X_s: A_ts (rt x rs) at a (stride lda), X_s (rs x n) and B_t
+// (rt x n) rows of B (stride ldb).
+template <typename T>
+static int trsm_update(const TrsmCtx<T>& c, size_t rt, size_t rs, const T* a, const T* xs, T* bt) {
+  int s;
+  if ((s = copy2d(c.ws_a, rs, a, c.lda, rt, rs, c.st))) return s;
+  const T* x = xs;
+  T* bb = bt;
+  if (c.ldb != (size_t)c.n) {
+    if ((s = copy2d(c.ws_x, (size_t)c.n, xs, c.ldb, rs, (size_t)c.n, c.st))) return s;
+    if ((s = copy2d(c.ws_b, (size_t)c.n, bt, c.ldb, rt, (size_t)c.n, c.st))) return s;
+    x = c.ws_x;
+    bb = c.ws_b;
+  }
+  if ((s = gemm_dev<T>(rt, c.n, rs, T(-1), c.ws_a, x, T(1), bb, c.st))) return s;
+  if (c.ldb != (size_t)c.n) return copy2d(bt, c.ldb, c.ws_b, (size_t)c.n, rt, (size_t)c.n, c.st);
+  return FERRAY_OK;
+}
+
+template <typename T>
+static int trsm_rec(const TrsmCtx<T>& c, size_t m, const T* a, T* b) {
+  if (m <= (size_t)TRSM_NB) {
+    const unsigned blocks = (unsigned)cdiv((size_t)c.n, 256);
+    if (c.upper)
+      hipLaunchKernelGGL((trsm_leaf_kernel<T, false>), dim3(blocks), dim3(256), 0, c.st, (int)m,
+                         c.n, a, c.lda, b, c.ldb, (int)c.unit);
+    else
+      hipLaunchKernelGGL((trsm_leaf_kernel<T, true>), dim3(blocks), dim3(256), 0, c.st, (int)m,
+                         c.n, a, c.lda, b, c.ldb, (int)c.unit);
+    FG_CHECK(hipGetLastError());
+    return FERRAY_OK;
+  }
+  const size_t m1 = trsm_split(m), m2 = m - m1;
+  int s;
+  if (!c.upper) {
+    // L11 X1 = B1; B2 -= L21 X1; L22 X2 = B2
+    if ((s = trsm_rec(c, m1, a, b))) return s;
+    if ((s = trsm_update(c, m2, m1, a + m1 * c.lda, b, b + m1 * c.ldb))) return s;
+    return trsm_rec(c, m2, a + m1 * c.lda + m1, b + m1 * c.ldb);
+  }
+  // U22 X2 = B2; B1 -= U12 X2; U11 X1 = B1
+  if ((s = trsm_rec(c, m2, a + m1 * c.lda + m1, b + m1 * c.ldb))) return s;
+  if ((s = trsm_update(c, m1, m2, a + m1, b + m1 * c.ldb, b))) return s;
+  return trsm_rec(c, m1, a, b);
+}
+
+template <typename T>
+static int trsm_dev(bool upper, size_t m, size_t n, T alpha, const T* a, size_t lda, T* b,
+                    size_t ldb, bool unit_diag, hipStream_t st) {
+  if (m == 0 || n == 0) return FERRAY_OK;
+  if (!a || !b || lda < m || ldb < n) return FERRAY_ERR_INVALID;
+  if (m > (1u << 30) || n > (1u << 30)) return FERRAY_ERR_INVALID;
+  if (alpha != T(1)) {
+    const size_t blocks = std::min<size_t>(cdiv(m * n, 256), 65536);
+    hipLaunchKernelGGL((trsm_scale_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, (int)m,
+                       (int)n, alpha, b, ldb);
+    FG_CHECK(hipGetLastError());
+  }
+  TrsmCtx<T> c{(int)n, lda, ldb, upper, unit_diag, nullptr, nullptr, nullptr, st};
+  if (m > (size_t)TRSM_NB) {
+    // The top split has the largest blocks.
+    const size_t m1 = trsm_split(m), m2 = m - m1;
+    const size_t blk = m1 * m2;
+    const size_t pack = ldb != n ? m * n : 0;
+    const size_t a_bytes = (blk * sizeof(T) + 255) / 256 * 256;
+    void* ws = nullptr;
+    int s = workspace(st, a_bytes + 2 * pack * sizeof(T) + 256, &ws);
+    if (s) return s;
+    c.ws_a = reinterpret_cast<T*>(ws);
+    c.ws_x = reinterpret_cast<T*>(reinterpret_cast<uint8_t*>(ws) + a_bytes);
+    c.ws_b = c.ws_x + pack;
+  }
+  return trsm_rec(c, m, a, b);
+}
+
+}  // namespace fg
+
+using namespace fg;
+
+extern "C" {
+
+int ferray_trsm_dev(int dtype, int upper, size_t m, size_t n, double alpha, const void* a,
+                    size_t lda, void* b, size_t ldb, bool unit_diag, void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  hipStream_t st = resolve_stream(stream);
+  if (dtype == FERRAY_DTYPE_F64)
+    return trsm_dev<double>(upper != 0, m, n, alpha, static_cast<const double*>(a), lda,
+                            static_cast<double*>(b), ldb, unit_diag, st);
+  if (dtype == FERRAY_DTYPE_F32)
+    return trsm_dev<float>(upper != 0, m, n, (float)alpha, static_cast<const float*>(a), lda,
+                           static_cast<float*>(b), ldb, unit_diag, st);
+  return FERRAY_ERR_INVALID;
+}
+
+}  // extern "C"

@@ -22,7 +22,8 @@ __all__ = [
     "gpu_supports", "gemm_f64", "gemm_f32", "gemm_c64", "gemm_c32", "gemm_i8",
     "gemm_i8_signed", "gemm_i16", "quantized_matmul", "gemm_bf16_f32",
     "gemm_f16_f32", "dgemm_ozaki", "last_status", "DeviceBuffer", "fill", "sync",
-    "sgemm_bf16x3", "FILL_F64_UNIFORM", "FILL_F64_NORMAL", "FILL_F64_WIDE", "FILL_F32_UNIFORM",
+    "sgemm_bf16x3", "trsm_left_lower_f64", "trsm_left_upper_f64",
+    "trsm_left_lower_f32", "trsm_left_upper_f32", "FILL_F64_UNIFORM", "FILL_F64_NORMAL", "FILL_F64_WIDE", "FILL_F32_UNIFORM",
     "FILL_F32_NORMAL", "FILL_I8", "FILL_I16", "FILL_U8", "Event", "device_sync",
 ]
 
@@ -78,6 +79,38 @@ def gemm_f32(m, n, k, alpha, a, b, beta, c) -> bool:
     _need(a, np.float32, m * k, "a"); _need(b, np.float32, k * n, "b")
     _need(c, np.float32, m * n, "c", True)
     return bool(lib().ferray_gemm_f32(m, n, k, alpha, _ptr(a), _ptr(b), beta, _ptr(c)))
+
+
+def _trsm(fn, dtype, m, n, alpha, a, lda, b, ldb, unit_diag) -> bool:
+    if lda < m or ldb < n:
+        raise ValueError("lda must be >= m and ldb >= n")
+    _need(a, dtype, m * lda if m else 0, "a")
+    _need(b, dtype, m * ldb if m else 0, "b", True)
+    return bool(fn(m, n, alpha, _ptr(a), lda, _ptr(b), ldb, bool(unit_diag)))
+
+
+def trsm_left_lower_f64(m, n, alpha, a, lda, b, ldb, unit_diag) -> bool:
+    """L X = alpha B in place, L lower triangular (trsm.rs:46-68)."""
+    return _trsm(lib().ferray_trsm_left_lower_f64, np.float64, m, n, alpha, a, lda, b, ldb,
+                 unit_diag)
+
+
+def trsm_left_upper_f64(m, n, alpha, a, lda, b, ldb, unit_diag) -> bool:
+    """U X = alpha B in place, U upper triangular (trsm.rs:70-92)."""
+    return _trsm(lib().ferray_trsm_left_upper_f64, np.float64, m, n, alpha, a, lda, b, ldb,
+                 unit_diag)
+
+
+def trsm_left_lower_f32(m, n, alpha, a, lda, b, ldb, unit_diag) -> bool:
+    """f32 sibling (trsm.rs:94-116)."""
+    return _trsm(lib().ferray_trsm_left_lower_f32, np.float32, m, n, alpha, a, lda, b, ldb,
+                 unit_diag)
+
+
+def trsm_left_upper_f32(m, n, alpha, a, lda, b, ldb, unit_diag) -> bool:
+    """f32 sibling (trsm.rs:118-140)."""
+    return _trsm(lib().ferray_trsm_left_upper_f32, np.float32, m, n, alpha, a, lda, b, ldb,
+                 unit_diag)
 
 
 def sgemm_bf16x3(m, n, k, alpha, a, b, beta, c, terms: int = 3) -> bool:

@@ -120,6 +120,32 @@ int ferray_gemm_f16_f32_dev(size_t m, size_t n, size_t k, const uint16_t *a,
                             const uint16_t *b, float *c, bool accumulate,
                             void *stream);
 
+/* ---------------- triangular solve (GEMM consumer) ---------------- */
+
+/* Solves L X = alpha B in place: L lower triangular m x m (row stride
+ * lda >= m), B m x n (row stride ldb >= n); unit_diag ignores the stored
+ * diagonal.  Replaces trsm::trsm_left_lower_f64 (trsm.rs:46-68). */
+bool ferray_trsm_left_lower_f64(size_t m, size_t n, double alpha,
+                                const double *a, size_t lda, double *b,
+                                size_t ldb, bool unit_diag);
+/* U X = alpha B, U upper triangular.  Replaces trsm::trsm_left_upper_f64
+ * (trsm.rs:70-92). */
+bool ferray_trsm_left_upper_f64(size_t m, size_t n, double alpha,
+                                const double *a, size_t lda, double *b,
+                                size_t ldb, bool unit_diag);
+/* f32 siblings.  Replace trsm::trsm_left_lower_f32 / trsm_left_upper_f32
+ * (trsm.rs:94-140). */
+bool ferray_trsm_left_lower_f32(size_t m, size_t n, float alpha,
+                                const float *a, size_t lda, float *b,
+                                size_t ldb, bool unit_diag);
+bool ferray_trsm_left_upper_f32(size_t m, size_t n, float alpha,
+                                const float *a, size_t lda, float *b,
+                                size_t ldb, bool unit_diag);
+/* Device form: dtype FERRAY_DTYPE_F64 or F32, upper 0/1. */
+int ferray_trsm_dev(int dtype, int upper, size_t m, size_t n, double alpha,
+                    const void *a, size_t lda, void *b, size_t ldb,
+                    bool unit_diag, void *stream);
+
 /* ---------------- split-precision DGEMM ---------------- */
 
 /* C = alpha*A@B + beta*C in f64 computed on INT8 matrix cores by the

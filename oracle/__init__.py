@@ -45,6 +45,10 @@ _SIGS = {
     "orc_gemm_f32_goto": (_i, [_sz, _sz, _sz, _f, _p, _p, _f, _p, _i]),
     "orc_gemm_i8_signed_threaded": (_i, [_sz, _sz, _sz, _p, _p, _p, _i]),
     "orc_cpu_has_avx2_fma": (_i, []),
+    "orc_trsm_left_lower_f64": (None, [_sz, _sz, _d, _p, _sz, _p, _sz, _i]),
+    "orc_trsm_left_upper_f64": (None, [_sz, _sz, _d, _p, _sz, _p, _sz, _i]),
+    "orc_trsm_left_lower_f32": (None, [_sz, _sz, _f, _p, _sz, _p, _sz, _i]),
+    "orc_trsm_left_upper_f32": (None, [_sz, _sz, _f, _p, _sz, _p, _sz, _i]),
 }
 _lib = None
 
@@ -203,3 +207,12 @@ def gemm_i8_signed_threaded(m, n, k, a, b, c, workers: int = 0) -> int:
 
 def cpu_has_avx2_fma() -> bool:
     return bool(lib().orc_cpu_has_avx2_fma())
+
+
+# ------------------------------------------------------------ triangular solve
+
+def trsm(lower: bool, m, n, alpha, a, lda, b, ldb, unit_diag) -> None:
+    """trsm.rs restated (NB = 32 block recursion, scalar updates); in place."""
+    f32 = a.dtype == np.float32
+    name = f"orc_trsm_left_{'lower' if lower else 'upper'}_{'f32' if f32 else 'f64'}"
+    getattr(lib(), name)(m, n, alpha, _p_(a), lda, _p_(b), ldb, int(bool(unit_diag)))

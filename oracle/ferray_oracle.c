@@ -814,3 +814,96 @@ int orc_gemm_i8_signed_threaded(size_t m, size_t n, size_t k, const int8_t *a,
   for (int i = 0; i < nt; ++i) pthread_join(th[i], NULL);
   return workers;
 }
+
+/* ------------------------------------------------------------------ */
+/* Triangular solve (trsm.rs)                                           */
+/* ------------------------------------------------------------------ */
+/* Restates trsm.rs:46-140 (entry points: scale B by alpha when alpha != 1,
+ * then recurse), trsm.rs:165-200 / :264-301 (block recursion with NB = 32,
+ * m1 = min(round_up(m/2, 32), m - 32)), trsm.rs:205-225 (scalar trailing
+ * update B2 -= A21 X1, loops i, p, j) and trsm.rs:227-262 / :304-330 (the
+ * unblocked substitution: b[i,j] -= a[i,p] * b[p,j], then *= 1 / a[i,i]).
+ * Built with -ffp-contract=off, every step rounds like the reference's
+ * (Rust does not contract a*b - c). */
+#define ORC_TRSM_NB 32
+
+#define ORC_TRSM(T, SUF)                                                                       \
+  static void orc_trsm_scale_##SUF(size_t m, size_t n, T alpha, T *b, size_t ldb) {            \
+    for (size_t i = 0; i < m; ++i)                                                             \
+      for (size_t j = 0; j < n; ++j) b[i * ldb + j] *= alpha;                                  \
+  }                                                                                            \
+  static void orc_trsm_update_##SUF(size_t m2, size_t n, size_t k, const T *l, size_t lda,     \
+                                    const T *x, size_t ldb, T *b2) {                           \
+    for (size_t i = 0; i < m2; ++i)                                                            \
+      for (size_t p = 0; p < k; ++p) {                                                         \
+        const T lip = l[i * lda + p];                                                          \
+        for (size_t j = 0; j < n; ++j) b2[i * ldb + j] -= lip * x[p * ldb + j];                \
+      }                                                                                        \
+  }                                                                                            \
+  static void orc_trsm_lln_unb_##SUF(size_t m, size_t n, const T *a, size_t lda, T *b,         \
+                                     size_t ldb, int unit) {                                   \
+    for (size_t i = 0; i < m; ++i) {                                                           \
+      for (size_t p = 0; p < i; ++p) {                                                         \
+        const T aip = a[i * lda + p];                                                          \
+        for (size_t j = 0; j < n; ++j) b[i * ldb + j] -= aip * b[p * ldb + j];                 \
+      }                                                                                        \
+      if (!unit) {                                                                             \
+        const T inv = (T)1 / a[i * lda + i];                                                   \
+        for (size_t j = 0; j < n; ++j) b[i * ldb + j] *= inv;                                  \
+      }                                                                                        \
+    }                                                                                          \
+  }                                                                                            \
+  static void orc_trsm_lun_unb_##SUF(size_t m, size_t n, const T *a, size_t lda, T *b,         \
+                                     size_t ldb, int unit) {                                   \
+    for (size_t ii = m; ii-- > 0;) {                                                           \
+      for (size_t p = ii + 1; p < m; ++p) {                                                    \
+        const T aip = a[ii * lda + p];                                                         \
+        for (size_t j = 0; j < n; ++j) b[ii * ldb + j] -= aip * b[p * ldb + j];                \
+      }                                                                                        \
+      if (!unit) {                                                                             \
+        const T inv = (T)1 / a[ii * lda + ii];                                                 \
+        for (size_t j = 0; j < n; ++j) b[ii * ldb + j] *= inv;                                 \
+      }                                                                                        \
+    }                                                                                          \
+  }                                                                                            \
+  static size_t orc_trsm_split_##SUF(size_t m) {                                               \
+    size_t h = (m / 2 + ORC_TRSM_NB - 1) / ORC_TRSM_NB * ORC_TRSM_NB;                          \
+    return h < m - ORC_TRSM_NB ? h : m - ORC_TRSM_NB;                                          \
+  }                                                                                            \
+  static void orc_trsm_lln_rec_##SUF(size_t m, size_t n, const T *a, size_t lda, T *b,         \
+                                     size_t ldb, int unit) {                                   \
+    if (m <= ORC_TRSM_NB) {                                                                    \
+      orc_trsm_lln_unb_##SUF(m, n, a, lda, b, ldb, unit);                                      \
+      return;                                                                                  \
+    }                                                                                          \
+    const size_t m1 = orc_trsm_split_##SUF(m), m2 = m - m1;                                    \
+    orc_trsm_lln_rec_##SUF(m1, n, a, lda, b, ldb, unit);                                       \
+    orc_trsm_update_##SUF(m2, n, m1, a + m1 * lda, lda, b, ldb, b + m1 * ldb);                 \
+    orc_trsm_lln_rec_##SUF(m2, n, a + m1 * lda + m1, lda, b + m1 * ldb, ldb, unit);            \
+  }                                                                                            \
+  static void orc_trsm_lun_rec_##SUF(size_t m, size_t n, const T *a, size_t lda, T *b,         \
+                                     size_t ldb, int unit) {                                   \
+    if (m <= ORC_TRSM_NB) {                                                                    \
+      orc_trsm_lun_unb_##SUF(m, n, a, lda, b, ldb, unit);                                      \
+      return;                                                                                  \
+    }                                                                                          \
+    const size_t m1 = orc_trsm_split_##SUF(m), m2 = m - m1;                                    \
+    orc_trsm_lun_rec_##SUF(m2, n, a + m1 * lda + m1, lda, b + m1 * ldb, ldb, unit);            \
+    orc_trsm_update_##SUF(m1, n, m2, a + m1, lda, b + m1 * ldb, ldb, b);                       \
+    orc_trsm_lun_rec_##SUF(m1, n, a, lda, b, ldb, unit);                                       \
+  }                                                                                            \
+  void orc_trsm_left_lower_##SUF(size_t m, size_t n, T alpha, const T *a, size_t lda, T *b,    \
+                                 size_t ldb, int unit) {                                       \
+    if (m == 0 || n == 0) return;                                                              \
+    if (alpha != (T)1) orc_trsm_scale_##SUF(m, n, alpha, b, ldb);                              \
+    orc_trsm_lln_rec_##SUF(m, n, a, lda, b, ldb, unit);                                        \
+  }                                                                                            \
+  void orc_trsm_left_upper_##SUF(size_t m, size_t n, T alpha, const T *a, size_t lda, T *b,    \
+                                 size_t ldb, int unit) {                                       \
+    if (m == 0 || n == 0) return;                                                              \
+    if (alpha != (T)1) orc_trsm_scale_##SUF(m, n, alpha, b, ldb);                              \
+    orc_trsm_lun_rec_##SUF(m, n, a, lda, b, ldb, unit);                                        \
+  }
+
+ORC_TRSM(double, f64)
+ORC_TRSM(float, f32)

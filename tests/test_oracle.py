@@ -417,3 +417,92 @@ def test_double_double_reference_is_correctly_rounded():
             ex = sum(Fraction(x) * Fraction(y) for x, y in zip(A[i], B[:, j]))
             err = abs(Fraction(hi[i, j]) - ex)
             assert err <= abs(ex) * Fraction(2) ** -52 + Fraction(2) ** -1000
+
+
+# ------------------------------------------------------------------ trsm.rs
+
+def _ref_lower(m):
+    """make_lower_triangular of trsm.rs's tests."""
+    a = np.zeros((m, m))
+    for i in range(m):
+        for j in range(i + 1):
+            a[i, j] = 2.0 + i * 0.5 if i == j else 0.1 * (i + j + 1)
+    return a
+
+
+def _ref_upper(m):
+    a = np.zeros((m, m))
+    for i in range(m):
+        for j in range(i, m):
+            a[i, j] = 2.0 + i * 0.5 if i == j else 0.1 * (i + j + 1)
+    return a
+
+
+@pytest.mark.parametrize("lower,m,n,tol,gen", [
+    (True, 5, 3, 1e-10, lambda i: i * 0.7 + 1.0),                 # trsm_lln_f64_small
+    (True, 80, 7, 1e-9, lambda i: np.sin(i * 0.13) + 1.0),        # trsm_lln_f64_large_blocked
+    (False, 5, 3, 1e-10, lambda i: i * 0.7 + 1.0),                # trsm_lun_f64_small
+    (False, 80, 7, 1e-9, lambda i: np.sin(i * 0.13) + 1.0),       # trsm_lun_f64_large_blocked
+])
+def test_trsm_reference_cases(lower, m, n, tol, gen):
+    a = _ref_lower(m) if lower else _ref_upper(m)
+    b = np.array([gen(i) for i in range(m * n)], dtype=np.float64)
+    x = b.copy()
+    oracle.trsm(lower, m, n, 1.0, a, m, x, n, False)
+    assert np.max(np.abs(a @ x.reshape(m, n) - b.reshape(m, n))) < tol
+
+
+def test_trsm_reference_alpha_unit_and_f32():
+    a = _ref_lower(4)
+    b = np.arange(8, dtype=np.float64) + 1.0
+    x = b.copy()
+    oracle.trsm(True, 4, 2, 3.0, a, 4, x, 2, False)                 # trsm_alpha_scales_b
+    assert np.max(np.abs(a @ x.reshape(4, 2) - 3.0 * b.reshape(4, 2))) < 1e-10
+    a = _ref_lower(6)
+    np.fill_diagonal(a, 1.0)
+    b = np.arange(24) * 0.3 + 0.5
+    xu, xe = b.copy(), b.copy()
+    oracle.trsm(True, 6, 4, 1.0, a, 6, xu, 4, True)                 # trsm_lln_f64_unit_diag
+    oracle.trsm(True, 6, 4, 1.0, a, 6, xe, 4, False)
+    assert np.max(np.abs(xu - xe)) < 1e-12
+    for lower, m, n in ((True, 65, 4), (False, 70, 3)):             # trsm_l{l,u}n_f32_blocked
+        a = (_ref_lower(m) if lower else _ref_upper(m)).astype(np.float32)
+        b = ((np.arange(m * n) * (0.13 if lower else 0.07)) + (0.5 if lower else -0.5)).astype(np.float32)
+        x = b.copy()
+        oracle.trsm(lower, m, n, np.float32(1.0), a, m, x, n, False)
+        assert np.max(np.abs(a @ x.reshape(m, n) - b.reshape(m, n))) < 1e-3
+
+
+def test_trsm_exact_on_integer_systems():
+    r = rng(12)
+    m, n = 70, 5
+    for lower in (True, False):
+        a = np.tril(r.integers(-3, 4, (m, m))).astype(np.float64) if lower else \
+            np.triu(r.integers(-3, 4, (m, m))).astype(np.float64)
+        np.fill_diagonal(a, 1.0)
+        xs = r.integers(-5, 6, (m, n)).astype(np.float64)
+        # keep |B| small enough that every intermediate is an exact integer
+        b = a @ xs
+        if np.max(np.abs(b)) > 2 ** 40:
+            continue
+        x = b.ravel().copy()
+        oracle.trsm(lower, m, n, 1.0, a, m, x, n, True)
+        assert np.array_equal(x.reshape(m, n), xs)
+
+
+def test_trsm_lower_recursion_keeps_the_substitution_order():
+    # Per element the blocked lower solve subtracts a_ip x_p for p ascending
+    # and then scales, exactly like one unblocked sweep, so the recursion
+    # (split at multiples of 32, scalar updates) must not change any bit.
+    r = rng(13)
+    m, n = 150, 6
+    a = np.tril(r.uniform(-1, 1, (m, m))) + np.diag(np.full(m, 3.0))
+    b = r.standard_normal(m * n)
+    x = b.copy()
+    oracle.trsm(True, m, n, 1.0, a, m, x, n, False)
+    y = b.reshape(m, n).copy()
+    for i in range(m):
+        for p in range(i):
+            y[i] = y[i] - a[i, p] * y[p]
+        y[i] = y[i] * (1.0 / a[i, i])
+    assert x.reshape(m, n).tobytes() == y.tobytes()
