@@ -23,6 +23,10 @@ Besides the headline the line carries:
   sweep         - the other dtypes of the GEMM family at their config sizes
                   (f32 8192, c64/c32 4096, i8 8192, i16 4096, bf16 8192),
                   one timed number each (--no-sweep skips them);
+  consumers     - the product layer's GEMM consumers on the device: TRSM at
+                  8192 x 8192, 256 batched 256^3 products in one grouped
+                  launch against 256 launches, matrix_power (--no-consumers
+                  skips them);
   serving       - 20,000-request Poisson trace (M, N, K log-uniform in
                   [16, 2048], f64/f32/c64/i8 mix, operands resident in HBM)
                   through the serving loop with continuous batching and
@@ -169,6 +173,88 @@ def split_sgemm(fa, n: int, steps: int, warmup: int, samples: int = 16):
     return out
 
 
+def consumers(fa, steps: int, warmup: int):
+    """GEMM consumers of the product layer, device resident:
+      trsm      - L X = B, L lower 8192 x 8192, B 8192 x 8192 (f64): m^2 n flop
+      batched   - 256 independent 256^3 f64 products: one grouped launch
+                  (ferray_matmul_batched_dev) against 256 single launches
+      matrix_power - A^13 of a 4096 x 4096 f64 matrix (5 squarings/products)"""
+    L = fa.lib()
+    out = {}
+    m = n = 8192
+    dA, dB = fa.DeviceBuffer(m * m * 8), fa.DeviceBuffer(m * n * 8)
+    fa.fill(dA, m * m, fa.FILL_F64_UNIFORM, 31)
+    fa.fill(dB, m * n, fa.FILL_F64_NORMAL, 32)
+    # A well-conditioned lower triangle: the kernel only reads the lower part;
+    # a large diagonal keeps the solve stable for the timing.
+    a_h = dA.download(np.float64, m * m).reshape(m, m) / np.sqrt(m)
+    np.fill_diagonal(a_h, 4.0)
+    dA.upload(np.ascontiguousarray(a_h))
+    del a_h
+    b0 = fa.DeviceBuffer(m * n * 8)
+    L.ferray_dev_copy(b0.ptr, dB.ptr, m * n * 8, None)
+
+    def trsm_step():
+        L.ferray_dev_copy(dB.ptr, b0.ptr, m * n * 8, None)
+        st = L.ferray_trsm_dev(0, 0, m, n, 1.0, dA.ptr, m, dB.ptr, n, False, None)
+        if st != 0:
+            raise RuntimeError(f"ferray_trsm_dev status {st}")
+
+    def copy_step():
+        L.ferray_dev_copy(dB.ptr, b0.ptr, m * n * 8, None)
+
+    ms = time_steps(fa, trsm_step, max(2, steps // 4), 1)
+    ms_copy = time_steps(fa, copy_step, 3, 1)
+    t = ms - ms_copy
+    out["trsm_lower_f64_8192x8192"] = {"ms": round(t, 3), "tflops": round(m * m * n / (t * 1e-3) / 1e12, 2),
+                                       "note": "m^2 n flop; the B reset copy is timed separately and subtracted"}
+    log(f"trsm f64 8192x8192: {t:.2f} ms, {m * m * n / (t * 1e-3) / 1e12:.1f} TFLOP/s")
+    for buf in (dA, dB, b0):
+        buf.free()
+
+    batch, sz = 256, 256
+    e = sz * sz * 8
+    dA, dB, dC = fa.DeviceBuffer(batch * e), fa.DeviceBuffer(batch * e), fa.DeviceBuffer(batch * e)
+    fa.fill(dA, batch * sz * sz, fa.FILL_F64_NORMAL, 33)
+    fa.fill(dB, batch * sz * sz, fa.FILL_F64_NORMAL, 34)
+
+    def grouped_step():
+        st = L.ferray_matmul_batched_dev(0, batch, batch, batch, sz, sz, sz, dA.ptr, dB.ptr,
+                                         dC.ptr, None)
+        if st != 0:
+            raise RuntimeError(f"ferray_matmul_batched_dev status {st}")
+
+    def loop_step():
+        for i in range(batch):
+            L.ferray_gemm_f64_dev(sz, sz, sz, 1.0, dA.ptr + i * e, dB.ptr + i * e, 0.0,
+                                  dC.ptr + i * e, None)
+
+    flops = 2.0 * batch * sz ** 3
+    ms_g = time_steps(fa, grouped_step, max(3, steps), warmup)
+    ms_l = time_steps(fa, loop_step, max(3, steps), warmup)
+    out["batched_256x256cubed_f64"] = {
+        "grouped_ms": round(ms_g, 3), "grouped_tflops": round(flops / (ms_g * 1e-3) / 1e12, 2),
+        "per_matrix_launches_ms": round(ms_l, 3),
+        "per_matrix_tflops": round(flops / (ms_l * 1e-3) / 1e12, 2),
+        "speedup": round(ms_l / ms_g, 2),
+        "note": "the 256 single launches are issued from Python (ctypes), so they include "
+                "host call overhead, as a caller looping over matrices would pay"}
+    log(f"batched 256 x 256^3 f64: grouped {ms_g:.3f} ms, 256 launches {ms_l:.3f} ms")
+    for buf in (dA, dB, dC):
+        buf.free()
+
+    from ferray_amd import products
+    sz = 4096
+    a = np.eye(sz) * 0.5 + np.random.default_rng(35).standard_normal((sz, sz)) * (0.1 / np.sqrt(sz))
+    t0 = time.perf_counter()
+    products.matrix_power(a, 13)
+    w = time.perf_counter() - t0
+    out["matrix_power_4096_p13_f64"] = {"wall_ms": round(w * 1e3, 2), "gemms": 6,
+                                        "note": "includes the 128 MiB upload and download"}
+    log(f"matrix_power 4096^13: {w * 1e3:.1f} ms wall")
+    return out
+
+
 def cpu_baseline(n: int = 4096, reps: int = 3):
     import oracle
     a = oracle.uniform_f64(n * n, 1)
@@ -294,6 +380,7 @@ def main() -> int:
     ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-host", action="store_true")
+    ap.add_argument("--no-consumers", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 1 GiB device-to-device copy, the reference point "
                          "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
@@ -385,6 +472,8 @@ def main() -> int:
         result["sweep"] = run_sweep(fa, args.steps, args.warmup)
     if not args.no_split:
         result["split_sgemm"] = split_sgemm(fa, n, args.steps, args.warmup)
+    if not args.no_consumers:
+        result["consumers"] = consumers(fa, args.steps, args.warmup)
     if not args.no_serve:
         result["serving"] = serving(fa)
         b = result["serving"]["best_at_p99_target"]["grouped"]
