@@ -36,6 +36,8 @@ hipStream_t resolve_stream(void* stream);
 // Work ordered on `stream` may use the returned memory until the next
 // call for the same stream; growing synchronises that stream first.
 int workspace(hipStream_t stream, size_t bytes, void** out);
+// Frees the workspace of a stream that is about to be destroyed.
+void release_workspace(hipStream_t stream);
 
 // Runtime initialisation on the calling thread (device selection).
 int ensure_device();

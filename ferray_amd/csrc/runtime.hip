@@ -20,7 +20,10 @@ int g_device = 0;
 struct ThreadState {
   hipStream_t stream = nullptr;
   ~ThreadState() {
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream) {
+      release_workspace(stream);
+      (void)hipStreamDestroy(stream);
+    }
   }
 };
 thread_local ThreadState t_state;
@@ -71,6 +74,17 @@ hipStream_t resolve_stream(void* stream) {
       t_state.stream = nullptr;
   }
   return t_state.stream;
+}
+
+void release_workspace(hipStream_t stream) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  auto it = g_ws.find(stream);
+  if (it == g_ws.end()) return;
+  if (it->second.ptr) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(it->second.ptr);
+  }
+  g_ws.erase(it);
 }
 
 int workspace(hipStream_t stream, size_t bytes, void** out) {

@@ -512,7 +512,10 @@ int ferray_server_destroy(ferray_server* s) {
     if (l.stream) (void)hipStreamSynchronize(l.stream);
     if (l.plan) ferray_grouped_plan_destroy(l.plan);
     if (l.ev) (void)hipEventDestroy(l.ev);
-    if (l.stream) (void)hipStreamDestroy(l.stream);
+    if (l.stream) {
+      release_workspace(l.stream);
+      (void)hipStreamDestroy(l.stream);
+    }
   }
   for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
   s->arena.destroy();
