@@ -21,7 +21,8 @@ Besides the headline the line carries:
                   splitting (3 and 6 piece products) against native FP32
                   MFMA: time, speed-up and sampled normwise error;
   sweep         - the other dtypes of the GEMM family at their config sizes
-                  (f32 8192, c64/c32 4096, i8 8192, i16 4096, bf16 8192),
+                  (f32 8192, c64/c32 4096, i8 8192, i16 4096, bf16/f16
+                  8192, OCP E4M3 fp8 8192, quantized_matmul u8 x i8 8192),
                   one timed number each (--no-sweep skips them);
   consumers     - the product layer's GEMM consumers on the device: TRSM at
                   8192 x 8192, 256 batched 256^3 products in one grouped
@@ -50,7 +51,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("GEMM TFLOP/s at 8192 (f64 native vs split-precision, % peak); "
           "serving p99 ms")
 PEAK = {  # MI355X published dense peaks, TFLOP/s (TOP/s for int8)
-    "f64": 78.6, "f32": 157.3, "bf16": 2500.0, "f16": 2500.0, "i8": 5000.0,
+    "f64": 78.6, "f32": 157.3, "bf16": 2500.0, "f16": 2500.0, "i8": 5000.0, "fp8": 5000.0,
 }
 HBM_PEAK_GBS = 8000.0
 
@@ -279,10 +280,14 @@ def run_sweep(fa, steps: int, warmup: int):
     out = {}
     cases = [("sgemm_f32_8192", "f32", 8192), ("zgemm_c64_4096", "c64", 4096),
              ("cgemm_c32_4096", "c32", 4096), ("igemm_i8_8192", "i8", 8192),
-             ("igemm_i16_4096", "i16", 4096), ("hgemm_bf16_8192", "bf16", 8192)]
+             ("igemm_i16_4096", "i16", 4096), ("hgemm_bf16_8192", "bf16", 8192),
+             ("hgemm_f16_8192", "f16", 8192), ("fp8gemm_e4m3_8192", "fp8", 8192),
+             ("quantized_matmul_8192", "quant", 8192)]
     for name, dt, n in cases:
-        esz = {"f32": 4, "c64": 16, "c32": 8, "i8": 1, "i16": 2, "bf16": 2}[dt]
-        csz = {"f32": 4, "c64": 16, "c32": 8, "i8": 4, "i16": 4, "bf16": 4}[dt]
+        esz = {"f32": 4, "c64": 16, "c32": 8, "i8": 1, "i16": 2, "bf16": 2, "f16": 2, "fp8": 1,
+               "quant": 1}[dt]
+        csz = {"f32": 4, "c64": 16, "c32": 8, "i8": 4, "i16": 4, "bf16": 4, "f16": 4, "fp8": 4,
+               "quant": 4}[dt]
         dA, dB, dC = fa.DeviceBuffer(n * n * esz), fa.DeviceBuffer(n * n * esz), \
             fa.DeviceBuffer(n * n * csz)
         if dt in ("f32", "c32"):
@@ -290,12 +295,14 @@ def run_sweep(fa, steps: int, warmup: int):
             fa.fill(dA, cnt, fa.FILL_F32_NORMAL, 11); fa.fill(dB, cnt, fa.FILL_F32_NORMAL, 12)
         elif dt == "c64":
             fa.fill(dA, 2 * n * n, fa.FILL_F64_NORMAL, 11); fa.fill(dB, 2 * n * n, fa.FILL_F64_NORMAL, 12)
-        elif dt == "i8":
+        elif dt in ("i8", "quant"):
             fa.fill(dA, n * n, fa.FILL_I8, 11, -128, 127); fa.fill(dB, n * n, fa.FILL_I8, 12, -128, 127)
+        elif dt == "fp8":  # finite positive E4M3 codes (exponent field <= 14)
+            fa.fill(dA, n * n, fa.FILL_I8, 11, 0, 0x77); fa.fill(dB, n * n, fa.FILL_I8, 12, 0, 0x77)
         elif dt == "i16":
             fa.fill(dA, n * n, fa.FILL_I16, 11, -32768, 32767)
             fa.fill(dB, n * n, fa.FILL_I16, 12, -32768, 32767)
-        else:  # bf16: random bit patterns with a bounded exponent
+        else:  # bf16 / f16: random bit patterns with a bounded exponent
             fa.fill(dA, n * n, fa.FILL_I16, 11, 0x3c00, 0x3fff)
             fa.fill(dB, n * n, fa.FILL_I16, 12, 0xbc00, 0xbfff)
         L = fa.lib()
@@ -306,6 +313,10 @@ def run_sweep(fa, steps: int, warmup: int):
             "i8": lambda: L.ferray_gemm_i8_signed_dev(n, n, n, dA.ptr, dB.ptr, dC.ptr, False, None),
             "i16": lambda: L.ferray_gemm_i16_dev(n, n, n, dA.ptr, dB.ptr, dC.ptr, False, None),
             "bf16": lambda: L.ferray_gemm_bf16_f32_dev(n, n, n, dA.ptr, dB.ptr, dC.ptr, False, None),
+            "f16": lambda: L.ferray_gemm_f16_f32_dev(n, n, n, dA.ptr, dB.ptr, dC.ptr, False, None),
+            "fp8": lambda: L.ferray_gemm_e4m3_f32_dev(n, n, n, dA.ptr, dB.ptr, dC.ptr, False, None),
+            "quant": lambda: L.ferray_quantized_matmul_dev(n, n, n, dA.ptr, 128, 0.02, dB.ptr, -3,
+                                                           0.05, dC.ptr, False, None),
         }
         fn = fns[dt]
         st = fn()
@@ -315,10 +326,10 @@ def run_sweep(fa, steps: int, warmup: int):
         ms = time_steps(fa, fn, max(3, steps // 2), max(1, warmup))
         flops = (8.0 if dt in ("c64", "c32") else 2.0) * n ** 3
         rate = flops / (ms * 1e-3) / 1e12
-        peak_key = {"c64": "f64", "c32": "f32", "i16": "i8"}.get(dt, dt)
+        peak_key = {"c64": "f64", "c32": "f32", "i16": "i8", "quant": "i8"}.get(dt, dt)
         peak = PEAK[peak_key] / (4.0 if dt == "i16" else 1.0)  # i16 = 4 int8 products
         out[name] = {"ms": round(ms, 4), "rate": round(rate, 2),
-                     "unit": "TOP/s" if dt in ("i8", "i16") else "TFLOP/s",
+                     "unit": "TOP/s" if dt in ("i8", "i16", "quant") else "TFLOP/s",
                      "frac_of_peak": round(rate / peak, 4)}
         log(f"{name}: {ms:.3f} ms  {rate:.1f}  ({rate / peak * 100:.1f}% of peak)")
         for buf in (dA, dB, dC):

@@ -36,6 +36,8 @@ SIGNATURES: dict[str, tuple] = {
     "ferray_quantized_matmul": (_b, [_sz, _sz, _sz, _p, _u8, _f, _p, _i8, _f, _p, _b]),
     "ferray_gemm_bf16_f32": (_b, [_sz, _sz, _sz, _p, _p, _p, _b]),
     "ferray_gemm_f16_f32": (_b, [_sz, _sz, _sz, _p, _p, _p, _b]),
+    "ferray_gemm_e4m3_f32": (_b, [_sz, _sz, _sz, _p, _p, _p, _b]),
+    "ferray_gemm_e5m2_f32": (_b, [_sz, _sz, _sz, _p, _p, _p, _b]),
     "ferray_gpu_available": (_b, []),
     "ferray_gemm_last_status": (_i, []),
     "ferray_status_string": (C.c_char_p, [_i]),
@@ -49,6 +51,8 @@ SIGNATURES: dict[str, tuple] = {
     "ferray_quantized_matmul_dev": (_i, [_sz, _sz, _sz, _p, _u8, _f, _p, _i8, _f, _p, _b, _p]),
     "ferray_gemm_bf16_f32_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
     "ferray_gemm_f16_f32_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
+    "ferray_gemm_e4m3_f32_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
+    "ferray_gemm_e5m2_f32_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
     "ferray_dgemm_ozaki": (_b, [_sz, _sz, _sz, _d, _p, _p, _d, _p, _i]),
     "ferray_dgemm_ozaki_dev": (_i, [_sz, _sz, _sz, _d, _p, _p, _d, _p, _i, _p]),
     "ferray_ozaki_max_moduli": (_i, []),

@@ -7,6 +7,9 @@
 //   i16 x i16 -> i32  four i8 passes over byte planes                 (gemm_i16)
 //   quantized_matmul  the u8 x i8 pass with a fused zero-point / scale epilogue
 //   bf16/f16 -> f32   one half-precision pass                 (gemm_{bf16,f16}_f32)
+//   fp8 -> f32        one pass on the 64-deep f8f6f4 instruction, OCP E4M3 or
+//                     E5M2 bytes (no reference entry point; the fp8 member of
+//                     the same family)
 //
 // B is transposed into k-contiguous planes by a prepass (which also forms the
 // i16 byte planes and the wrapping column sums); A is read in place when its
@@ -26,7 +29,10 @@
 
 namespace fg {
 
-enum NtKind { NT_I8S = 0, NT_U8 = 1, NT_I16 = 2, NT_BF16 = 3, NT_F16 = 4, NT_QUANT = 5 };
+enum NtKind {
+  NT_I8S = 0, NT_U8 = 1, NT_I16 = 2, NT_BF16 = 3, NT_F16 = 4, NT_QUANT = 5, NT_E4M3 = 6,
+  NT_E5M2 = 7
+};
 
 constexpr size_t NT_K_CHUNK = 1u << 16;
 
@@ -234,6 +240,7 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
                         void* c, bool accumulate, uint8_t a_zp, float scale_a, int8_t b_zp,
                         float scale_b, hipStream_t st) {
   const bool half = (kind == NT_BF16 || kind == NT_F16);
+  const bool fp8 = (kind == NT_E4M3 || kind == NT_E5M2);
   const bool is_u8 = (kind == NT_U8 || kind == NT_QUANT);
   const size_t es = (half || kind == NT_I16) ? 2 : 1;  // input element bytes
   const size_t pe = half ? 2 : 1;                       // plane element bytes
@@ -327,7 +334,7 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
   const int M = (int)m, N = (int)n;
   // Integer paths reduce K in chunks so no i32 accumulator can overflow.
   const size_t kb_total = Kp * pe;
-  const size_t chunk = half ? kb_total : align_up(NT_K_CHUNK, S::KB);
+  const size_t chunk = (half || fp8) ? kb_total : align_up(NT_K_CHUNK, S::KB);
   for (size_t kb0 = 0; kb0 < kb_total; kb0 += chunk) {
     NtOperands q = p;
     q.a = p.a + kb0;
@@ -340,6 +347,10 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
       EpiF32 e{reinterpret_cast<float*>(c), M, N, read_c};
       rc = kind == NT_BF16 ? launch_mfma_nt<ELEM_BF16, S, false>(q, 1, e, st)
                            : launch_mfma_nt<ELEM_F16, S, false>(q, 1, e, st);
+    } else if (fp8) {
+      EpiF32 e{reinterpret_cast<float*>(c), M, N, read_c};
+      rc = kind == NT_E4M3 ? launch_mfma_nt<ELEM_FP8, S, false>(q, 1, e, st)
+                           : launch_mfma_nt<ELEM_BF8, S, false>(q, 1, e, st);
     } else if (kind == NT_QUANT) {
       if (first) {
         EpiQuant e{reinterpret_cast<float*>(c), M, N, accumulate ? 1 : 0, csp, rsp,
@@ -502,6 +513,20 @@ int ferray_gemm_f16_f32_dev(size_t m, size_t n, size_t k, const uint16_t* a, con
   int s = ensure_device();
   if (s) return s;
   return run_nt(NT_F16, m, n, k, a, b, c, accumulate, 0, 1.f, 0, 1.f, resolve_stream(stream));
+}
+
+int ferray_gemm_e4m3_f32_dev(size_t m, size_t n, size_t k, const uint8_t* a, const uint8_t* b,
+                             float* c, bool accumulate, void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  return run_nt(NT_E4M3, m, n, k, a, b, c, accumulate, 0, 1.f, 0, 1.f, resolve_stream(stream));
+}
+
+int ferray_gemm_e5m2_f32_dev(size_t m, size_t n, size_t k, const uint8_t* a, const uint8_t* b,
+                             float* c, bool accumulate, void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  return run_nt(NT_E5M2, m, n, k, a, b, c, accumulate, 0, 1.f, 0, 1.f, resolve_stream(stream));
 }
 
 }  // extern "C"

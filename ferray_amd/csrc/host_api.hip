@@ -267,4 +267,26 @@ bool ferray_gemm_f16_f32(size_t m, size_t n, size_t k, const uint16_t* a, const 
                        }));
 }
 
+bool ferray_gemm_e4m3_f32(size_t m, size_t n, size_t k, const uint8_t* a, const uint8_t* b,
+                          float* c, bool accumulate) {
+  if (m == 0 || n == 0) return finish(FERRAY_OK);
+  return finish(staged(a, m * k, b, k * n, c, m * n * 4, accumulate,
+                       [&](void* da, void* db, void* dc, hipStream_t st) {
+                         return ferray_gemm_e4m3_f32_dev(m, n, k, (const uint8_t*)da,
+                                                         (const uint8_t*)db, (float*)dc,
+                                                         accumulate, st);
+                       }));
+}
+
+bool ferray_gemm_e5m2_f32(size_t m, size_t n, size_t k, const uint8_t* a, const uint8_t* b,
+                          float* c, bool accumulate) {
+  if (m == 0 || n == 0) return finish(FERRAY_OK);
+  return finish(staged(a, m * k, b, k * n, c, m * n * 4, accumulate,
+                       [&](void* da, void* db, void* dc, hipStream_t st) {
+                         return ferray_gemm_e5m2_f32_dev(m, n, k, (const uint8_t*)da,
+                                                         (const uint8_t*)db, (float*)dc,
+                                                         accumulate, st);
+                       }));
+}
+
 }  // extern "C"

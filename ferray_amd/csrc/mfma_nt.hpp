@@ -5,6 +5,13 @@
 //
 //   i8:        v_mfma_i32_32x32x32_i8          (32 bytes of k per step, i32 acc)
 //   bf16/f16:  v_mfma_f32_32x32x16_{bf16,f16}  (same byte layout, f32 acc)
+//   fp8/bf8:   v_mfma_f32_32x32x64_f8f6f4      (64 bytes of k per step, f32 acc;
+//              OCP E4M3 / E5M2 bytes)
+//
+// A k-step's fragment for lane l is row l%32 of the panel: the 16 bytes at
+// 16*(l/32) of each 32-byte k group (one group for i8/bf16/f16, two for fp8).
+// A and B fragments are read with the same byte map, so the k pairing of the
+// two operands is the identity whatever order the matrix core sums in.
 //
 // Block: WM x WN waves, block tile BM x BN, each wave a (BM/WM) x (BN/WN)
 // sub-tile of 32x32 MFMA tiles.  A pipeline stage moves KB bytes of k of both
@@ -27,10 +34,11 @@
 namespace fg {
 
 typedef int i16v __attribute__((ext_vector_type(16)));
+typedef int i8v __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef unsigned u4v __attribute__((ext_vector_type(4)));
 
-enum MfmaElem { ELEM_I8 = 0, ELEM_BF16 = 1, ELEM_F16 = 2 };
+enum MfmaElem { ELEM_I8 = 0, ELEM_BF16 = 1, ELEM_F16 = 2, ELEM_FP8 = 3, ELEM_BF8 = 4 };
 
 struct NtOperands {
   const uint8_t* a;         // plane 0 of A; plane z starts at a + z * a_plane
@@ -41,17 +49,36 @@ struct NtOperands {
   int tiles_m, tiles_n;
 };
 
+// One 32-byte k group per step: a 16-byte fragment per lane.
+struct NtFrag16 {
+  using frag = i4;
+  static constexpr int KSTEP = 32, READS = 1;
+  static __device__ __forceinline__ frag read(const uint8_t* p) {
+    return *reinterpret_cast<const i4*>(p);
+  }
+};
+// Two 32-byte k groups per step: a 32-byte fragment per lane.
+struct NtFrag32 {
+  using frag = i8v;
+  static constexpr int KSTEP = 64, READS = 2;
+  static __device__ __forceinline__ frag read(const uint8_t* p) {
+    const i4 lo = *reinterpret_cast<const i4*>(p);
+    const i4 hi = *reinterpret_cast<const i4*>(p + 32);
+    return i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  }
+};
+
 template <int ELEM>
 struct NtAcc;
 template <>
-struct NtAcc<ELEM_I8> {
+struct NtAcc<ELEM_I8> : NtFrag16 {
   using type = i16v;
   static __device__ __forceinline__ type step(i4 a, i4 b, type c) {
     return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
   }
 };
 template <>
-struct NtAcc<ELEM_BF16> {
+struct NtAcc<ELEM_BF16> : NtFrag16 {
   using type = f16v;
   static __device__ __forceinline__ type step(i4 a, i4 b, type c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a),
@@ -59,11 +86,27 @@ struct NtAcc<ELEM_BF16> {
   }
 };
 template <>
-struct NtAcc<ELEM_F16> {
+struct NtAcc<ELEM_F16> : NtFrag16 {
   using type = f16v;
   static __device__ __forceinline__ type step(i4 a, i4 b, type c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, a),
                                                   __builtin_bit_cast(h8, b), c, 0, 0, 0);
+  }
+};
+// Format operands of the f8f6f4 instruction: 0 = E4M3, 1 = E5M2.  Scale
+// operands 0 select the unscaled form.
+template <>
+struct NtAcc<ELEM_FP8> : NtFrag32 {
+  using type = f16v;
+  static __device__ __forceinline__ type step(i8v a, i8v b, type c) {
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);
+  }
+};
+template <>
+struct NtAcc<ELEM_BF8> : NtFrag32 {
+  using type = f16v;
+  static __device__ __forceinline__ type step(i8v a, i8v b, type c) {
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 1, 1, 0, 0, 0, 0);
   }
 };
 
@@ -99,8 +142,10 @@ __global__ __launch_bounds__(S::THREADS) void mfma_nt_kernel(NtOperands p, Epi e
   constexpr int TI = BM / S::WM / 32, TJ = BN / S::WN / 32;
   constexpr int CPR = KB / 16;  // 16-byte chunks per row per stage
   constexpr int AV = BM * CPR / NT, BV = BN * CPR / NT;
-  constexpr int NKK = KB / 32;
-  constexpr int NMF = TI * TJ, NRD = TI + TJ, NWR = AV + BV;
+  constexpr int KSTEP = Acc::KSTEP;
+  constexpr int NKK = KB / KSTEP;
+  constexpr int NMF = TI * TJ, NRD = (TI + TJ) * Acc::READS, NWR = AV + BV;
+  static_assert(KB % KSTEP == 0, "stage depth");
   static_assert(AV >= 1 && BV >= 1 && BM * CPR % NT == 0 && BN * CPR % NT == 0, "loader");
   extern __shared__ __attribute__((aligned(16))) uint8_t nt_smem[];
   uint8_t* As = nt_smem;
@@ -155,15 +200,15 @@ __global__ __launch_bounds__(S::THREADS) void mfma_nt_kernel(NtOperands p, Epi e
     const uint8_t* as = As + cur * BM * PITCH;
     const uint8_t* bs = Bs + cur * BN * PITCH;
     if (more) load(kb_next);
-    i4 af[NKK][TI], bf[NKK][TJ];
+    typename Acc::frag af[NKK][TI], bf[NKK][TJ];
 #pragma unroll
     for (int kk = 0; kk < NKK; ++kk) {
 #pragma unroll
       for (int i = 0; i < TI; ++i)
-        af[kk][i] = *reinterpret_cast<const i4*>(as + (wm0 + i * 32 + lr) * PITCH + kk * 32 + lk * 16);
+        af[kk][i] = Acc::read(as + (wm0 + i * 32 + lr) * PITCH + kk * KSTEP + lk * 16);
 #pragma unroll
       for (int j = 0; j < TJ; ++j)
-        bf[kk][j] = *reinterpret_cast<const i4*>(bs + (wn0 + j * 32 + lr) * PITCH + kk * 32 + lk * 16);
+        bf[kk][j] = Acc::read(bs + (wn0 + j * 32 + lr) * PITCH + kk * KSTEP + lk * 16);
     }
 #pragma unroll
     for (int kk = 0; kk < NKK; ++kk)

@@ -21,7 +21,7 @@ from ._lib import check, lib
 __all__ = [
     "gpu_supports", "gemm_f64", "gemm_f32", "gemm_c64", "gemm_c32", "gemm_i8",
     "gemm_i8_signed", "gemm_i16", "quantized_matmul", "gemm_bf16_f32",
-    "gemm_f16_f32", "dgemm_ozaki", "last_status", "DeviceBuffer", "fill", "sync",
+    "gemm_f16_f32", "gemm_e4m3_f32", "gemm_e5m2_f32", "dgemm_ozaki", "last_status", "DeviceBuffer", "fill", "sync",
     "sgemm_bf16x3", "trsm_left_lower_f64", "trsm_left_upper_f64",
     "trsm_left_lower_f32", "trsm_left_upper_f32", "FILL_F64_UNIFORM", "FILL_F64_NORMAL", "FILL_F64_WIDE", "FILL_F32_UNIFORM",
     "FILL_F32_NORMAL", "FILL_I8", "FILL_I16", "FILL_U8", "Event", "device_sync",
@@ -196,6 +196,20 @@ def gemm_f16_f32(m, n, k, a, b, c, accumulate) -> bool:
     a = _half_bits(a, "a", m * k); b = _half_bits(b, "b", k * n)
     _need(c, np.float32, m * n, "c", True)
     return bool(lib().ferray_gemm_f16_f32(m, n, k, _ptr(a), _ptr(b), _ptr(c), bool(accumulate)))
+
+
+def gemm_e4m3_f32(m, n, k, a, b, c, accumulate) -> bool:
+    """OCP E4M3 x E4M3 -> f32; ``a``/``b`` hold the fp8 bytes as uint8."""
+    _need(a, np.uint8, m * k, "a"); _need(b, np.uint8, k * n, "b")
+    _need(c, np.float32, m * n, "c", True)
+    return bool(lib().ferray_gemm_e4m3_f32(m, n, k, _ptr(a), _ptr(b), _ptr(c), bool(accumulate)))
+
+
+def gemm_e5m2_f32(m, n, k, a, b, c, accumulate) -> bool:
+    """OCP E5M2 x E5M2 -> f32; ``a``/``b`` hold the fp8 bytes as uint8."""
+    _need(a, np.uint8, m * k, "a"); _need(b, np.uint8, k * n, "b")
+    _need(c, np.float32, m * n, "c", True)
+    return bool(lib().ferray_gemm_e5m2_f32(m, n, k, _ptr(a), _ptr(b), _ptr(c), bool(accumulate)))
 
 
 # ---------------------------------------------------------------------------

@@ -76,6 +76,15 @@ bool ferray_gemm_bf16_f32(size_t m, size_t n, size_t k, const uint16_t *a,
  * (mod.rs:906-933). */
 bool ferray_gemm_f16_f32(size_t m, size_t n, size_t k, const uint16_t *a,
                          const uint16_t *b, float *c, bool accumulate);
+/* fp8 x fp8 -> f32 on the matrix cores' 8-bit float path: OCP E4M3 (no
+ * infinities, 0x7f/0xff NaN, max 448) or OCP E5M2 bytes, row-major, C
+ * written or (accumulate) added.  No counterpart in the reference (its 8-bit
+ * entries are the integer ones); it is the fp8 member of the family, with
+ * the argument order and ownership of ferray_gemm_bf16_f32. */
+bool ferray_gemm_e4m3_f32(size_t m, size_t n, size_t k, const uint8_t *a,
+                          const uint8_t *b, float *c, bool accumulate);
+bool ferray_gemm_e5m2_f32(size_t m, size_t n, size_t k, const uint8_t *a,
+                          const uint8_t *b, float *c, bool accumulate);
 
 /* Device probe: the analogue of gemm::cpu_supports_avx2_fma
  * (mod.rs:118-188) — true iff a gfx950 device is usable. */
@@ -119,6 +128,12 @@ int ferray_gemm_bf16_f32_dev(size_t m, size_t n, size_t k, const uint16_t *a,
 int ferray_gemm_f16_f32_dev(size_t m, size_t n, size_t k, const uint16_t *a,
                             const uint16_t *b, float *c, bool accumulate,
                             void *stream);
+int ferray_gemm_e4m3_f32_dev(size_t m, size_t n, size_t k, const uint8_t *a,
+                             const uint8_t *b, float *c, bool accumulate,
+                             void *stream);
+int ferray_gemm_e5m2_f32_dev(size_t m, size_t n, size_t k, const uint8_t *a,
+                             const uint8_t *b, float *c, bool accumulate,
+                             void *stream);
 
 /* ---------------- triangular solve (GEMM consumer) ---------------- */
 

@@ -39,6 +39,9 @@ _SIGS = {
     "orc_f16_to_f32": (_f, [C.c_uint16]),
     "orc_gemm_bf16_f32": (None, [_sz, _sz, _sz, _p, _p, _p, _i]),
     "orc_gemm_f16_f32": (None, [_sz, _sz, _sz, _p, _p, _p, _i]),
+    "orc_e4m3_to_f64": (_d, [C.c_uint8]),
+    "orc_e5m2_to_f64": (_d, [C.c_uint8]),
+    "orc_gemm_fp8_f64": (None, [_sz, _sz, _sz, _p, _p, _i, _p, _i]),
     "orc_dd_gemm_f64_sampled": (None, [_sz, _sz, _sz, _p, _p, _sz, _p, _sz, _p, _p, _p]),
     "orc_dd_gemm_c64_sampled": (None, [_sz, _sz, _sz, _p, _p, _sz, _p, _sz, _p, _p]),
     "orc_gemm_f64_goto": (_i, [_sz, _sz, _sz, _d, _p, _p, _d, _p, _i]),
@@ -164,6 +167,25 @@ def bf16_to_f32(h: int) -> float:
 
 def f16_to_f32(h: int) -> float:
     return lib().orc_f16_to_f32(h)
+
+
+def e4m3_to_f64(x: int) -> float:
+    """OCP E4M3 byte -> value (no reference counterpart; OCP FP8 spec v1.0)."""
+    return lib().orc_e4m3_to_f64(x)
+
+
+def e5m2_to_f64(x: int) -> float:
+    """OCP E5M2 byte -> value (OCP FP8 spec v1.0)."""
+    return lib().orc_e5m2_to_f64(x)
+
+
+def gemm_fp8_f64(m, n, k, a, b, fmt: str, c, accumulate) -> None:
+    """C (+)= A B over fp8 bytes (fmt 'e4m3' or 'e5m2') summed in double; the
+    fp8 member of the family has no reference entry point, so this is the
+    exact product (to double rounding) the device result is measured against."""
+    assert c.dtype == np.float64
+    lib().orc_gemm_fp8_f64(m, n, k, _p_(a), _p_(b), 0 if fmt == "e4m3" else 1, _p_(c),
+                           int(accumulate))
 
 
 # ------------------------------------------------------------ error reference

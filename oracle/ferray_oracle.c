@@ -385,6 +385,46 @@ void orc_gemm_f16_f32(size_t m, size_t n, size_t k, const uint16_t *a,
   orc_half_gemm(m, n, k, a, b, c, accumulate, orc_f16_to_f32);
 }
 
+/* OCP 8-bit floats (OCP "8-bit Floating Point Specification", v1.0, 2023).
+ * E4M3: bias 7, no infinities, S.1111.111 is NaN, largest finite 448.
+ * E5M2: bias 15, IEEE-style: exponent 31 is Inf (mantissa 0) or NaN.
+ * Subnormals (exponent 0) are mantissa * 2^(1 - bias - mantissa bits). */
+double orc_e4m3_to_f64(uint8_t x) {
+  const int sign = x >> 7, e = (x >> 3) & 15, mant = x & 7;
+  double v;
+  if (e == 15 && mant == 7) return NAN;
+  if (e == 0) v = ldexp((double)mant, -9);
+  else v = ldexp(1.0 + mant / 8.0, e - 7);
+  return sign ? -v : v;
+}
+double orc_e5m2_to_f64(uint8_t x) {
+  const int sign = x >> 7, e = (x >> 2) & 31, mant = x & 3;
+  double v;
+  if (e == 31) v = mant ? NAN : INFINITY;
+  else if (e == 0) v = ldexp((double)mant, -16);
+  else v = ldexp(1.0 + mant / 4.0, e - 15);
+  return sign ? -v : v;
+}
+/* C (+)= A B with A, B fp8 bytes (format 0 = E4M3, 1 = E5M2), summed in
+ * double: every product of two fp8 values is exact in double, so the result
+ * is the exact product to double rounding of the sums. */
+void orc_gemm_fp8_f64(size_t m, size_t n, size_t k, const uint8_t *a, const uint8_t *b,
+                      int format, double *c, int accumulate) {
+  double lut[256];
+  for (int i = 0; i < 256; ++i)
+    lut[i] = format == 0 ? orc_e4m3_to_f64((uint8_t)i) : orc_e5m2_to_f64((uint8_t)i);
+  for (size_t i = 0; i < m; ++i) {
+    double *ci = c + i * n;
+    if (!accumulate)
+      for (size_t j = 0; j < n; ++j) ci[j] = 0.0;
+    for (size_t p = 0; p < k; ++p) {
+      const double av = lut[a[i * k + p]];
+      const uint8_t *bp = b + p * n;
+      for (size_t j = 0; j < n; ++j) ci[j] += av * lut[bp[j]];
+    }
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* Double-double reference for error measurement                        */
 /* ------------------------------------------------------------------ */

@@ -403,6 +403,58 @@ def test_bf16_gemm_is_widen_then_sgemm():
     assert np.all(np.abs(c.reshape(m, n) - ref) <= 1e-5 * np.maximum(np.abs(ref), 1.0))
 
 
+# ---------------------------------------------------------------- fp8
+
+def test_fp8_decode_matches_the_ocp_spec_anchors():
+    # values the OCP FP8 spec tabulates for each format
+    e4 = {0x00: 0.0, 0x38: 1.0, 0xb8: -1.0, 0x7e: 448.0, 0x08: 2.0 ** -6, 0x07: 7 * 2.0 ** -9,
+          0x01: 2.0 ** -9, 0x3c: 1.5, 0x40: 2.0}
+    for x, v in e4.items():
+        assert oracle.e4m3_to_f64(x) == v, hex(x)
+    assert np.isnan(oracle.e4m3_to_f64(0x7f)) and np.isnan(oracle.e4m3_to_f64(0xff))
+    e5 = {0x00: 0.0, 0x3c: 1.0, 0xbc: -1.0, 0x7b: 57344.0, 0x04: 2.0 ** -14, 0x01: 2.0 ** -16,
+          0x03: 3 * 2.0 ** -16, 0x3e: 1.5}
+    for x, v in e5.items():
+        assert oracle.e5m2_to_f64(x) == v, hex(x)
+    assert oracle.e5m2_to_f64(0x7c) == np.inf and oracle.e5m2_to_f64(0xfc) == -np.inf
+    assert all(np.isnan(oracle.e5m2_to_f64(x)) for x in (0x7d, 0x7e, 0x7f, 0xfd))
+
+
+def test_fp8_decode_matches_torch_on_every_code():
+    torch = pytest.importorskip("torch")
+    codes = np.arange(256, dtype=np.uint8)
+    for fmt, dt, dec in (("e4m3", torch.float8_e4m3fn, oracle.e4m3_to_f64),
+                         ("e5m2", torch.float8_e5m2, oracle.e5m2_to_f64)):
+        ref = torch.from_numpy(codes).view(dt).to(torch.float64).numpy()
+        mine = np.array([dec(int(x)) for x in codes])
+        nan = np.isnan(ref)
+        assert np.array_equal(np.isnan(mine), nan), fmt
+        assert np.array_equal(mine[~nan], ref[~nan]), fmt
+
+
+@pytest.mark.parametrize("fmt", ["e4m3", "e5m2"])
+def test_fp8_gemm_is_decode_then_exact_product(fmt):
+    r = rng(8)
+    m, n, k = 6, 7, 50
+    dec = oracle.e4m3_to_f64 if fmt == "e4m3" else oracle.e5m2_to_f64
+    lut = np.array([dec(i) for i in range(256)])
+    finite = np.flatnonzero(np.isfinite(lut)).astype(np.uint8)
+    a = r.choice(finite, m * k)
+    b = r.choice(finite, k * n)
+    c = np.zeros(m * n)
+    oracle.gemm_fp8_f64(m, n, k, a, b, fmt, c, False)
+    A = lut[a].reshape(m, k); B = lut[b].reshape(k, n)
+    exact = [[sum(Fraction(x) * Fraction(y) for x, y in zip(A[i], B[:, j])) for j in range(n)]
+             for i in range(m)]
+    scale = np.abs(A) @ np.abs(B)
+    for i in range(m):
+        for j in range(n):
+            assert abs(Fraction(c[i * n + j]) - exact[i][j]) <= Fraction(k * 2.0 ** -53 * scale[i, j])
+    c2 = np.full(m * n, 0.5)
+    oracle.gemm_fp8_f64(m, n, k, a, b, fmt, c2, True)
+    assert np.array_equal(c2, c + 0.5) or np.allclose(c2, c + 0.5, rtol=1e-15, atol=0)
+
+
 # ---------------------------------------------------------------- dd reference
 
 def test_double_double_reference_is_correctly_rounded():
