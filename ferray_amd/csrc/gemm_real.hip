@@ -20,7 +20,8 @@
 //
 // Block: 256 threads (4 waves, 2x2), block tile 128x128, BK = 16, each wave
 // owns a 64x64 sub-tile = 4x4 MFMA tiles of 16x16.  A and B panels go through
-// registers into double-buffered LDS (one barrier per k-step).  Tiles are
+// registers into double-buffered LDS (one barrier per k-step), with an
+// explicit instruction schedule for each stage (see the stage body).  Tiles are
 // ordered in groups of GROUP_M rows of tiles, and the linear block id is
 // remapped so each XCD (blocks are dealt round-robin to the 8 XCDs) works on
 // a contiguous range of that order, which keeps the A and B panels a group
@@ -183,12 +184,12 @@ void gemm_real_kernel(RealArgs<T> p) {
   store_tiles(0);
   __syncthreads();
 
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < nk;
+  // Stage body; `more` is a constant at each call, so the steady-state
+  // stages carry no branches.
+  auto stage = [&](int cur, bool more, int k_next) {
     if (more) {
-      load_a((kt + 1) * BK);
-      load_b((kt + 1) * BK);
+      load_a(k_next);
+      load_b(k_next);
     }
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
@@ -203,8 +204,53 @@ void gemm_real_kernel(RealArgs<T> p) {
         for (int j = 0; j < 4; ++j) acc[i][j] = MF::run(af[i], bf[j], acc[i][j]);
     }
     if (more) store_tiles(cur ^ 1);
+    {
+      // Pipeline order for the scheduler (masks: 0x020 VMEM read, 0x100 DS
+      // read, 0x200 DS write, 0x008 MFMA): the next stage's global loads,
+      // this stage's first fragment reads, MFMAs interleaved with the
+      // fragment reads of the following k-step, and the last k-step's MFMAs
+      // interleaved with the LDS writes of the next stage, so the stage tail
+      // (write, wait, barrier) overlaps matrix work.  Measured at 8192 (f64,
+      // f32) and 4096 (complex): 4-10 % faster than the compiler's own
+      // order, with identical results (only the instruction order changes).
+      // f64 (real and complex) also spreads the global loads over the first
+      // k-step's MFMAs (+5 % at 8192); f32 issues them up front (spreading
+      // measured 0.4 % slower there).
+      constexpr int NRD = 4, NWR = 8, NVM = AV + (CPLX ? CV : BV);
+      constexpr bool SPREAD = sizeof(T) == 8;
+      if (more && !SPREAD) __builtin_amdgcn_sched_group_barrier(0x020, NVM, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, NRD, 0);
+#pragma unroll
+      for (int kk = 0; kk < BK / 4 - 1; ++kk) {
+#pragma unroll
+        for (int q = 0; q < NRD; ++q) {
+          if (SPREAD && more && kk == 0) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+            __builtin_amdgcn_sched_group_barrier(0x020, NVM / NRD, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 16 / NRD - 2, 0);
+          } else {
+            __builtin_amdgcn_sched_group_barrier(0x008, 16 / NRD, 0);
+          }
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+      }
+      if (more) {
+#pragma unroll
+        for (int q = 0; q < NWR; ++q) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 16 / NWR, 0);
+          __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+        }
+      } else {
+        __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
+      }
+    }
+  };
+  int kt = 0;
+  for (; kt + 1 < nk; ++kt) {
+    stage(kt & 1, true, (kt + 1) * BK);
     __syncthreads();
   }
+  stage(kt & 1, false, 0);
 
   // Epilogue.  acc[i][j][r] holds C[m0+wm0+16i+row(lk,r)][n0+wn0+16j+lr].
 #pragma unroll

@@ -191,14 +191,17 @@ __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, 
 #pragma unroll
     for (int j = 0; j < TJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  const int nk = (K + BK - 1) / BK;
-  load(0);
-  store(0);
-  __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < nk;
-    if (more) load((kt + 1) * BK);
+  // Stage body; `more` is a constant at each call, so the steady-state
+  // stages carry no branches, and the scheduler is given the same pipeline
+  // order as the single-problem kernel: the next stage's global loads, the
+  // first fragment reads, MFMAs interleaved with the following k-step's
+  // reads, and the last k-step's MFMAs interleaved with the LDS writes.
+  constexpr int NMF = TI * TJ;
+  constexpr int NRD = (TI + TJ) / 2;                              // ds_read2 pairs
+  constexpr int NWR = AV * V / 2 + (CPLX ? BV * V : BV);          // LDS write instructions
+  constexpr int NVM = AV + BV;
+  auto stage = [&](int cur, bool more, int k_next) {
+    if (more) load(k_next);
     const T* as = As + cur * BK * AP;
     const T* bs = Bs + cur * BK * BN;
 #pragma unroll
@@ -214,8 +217,37 @@ __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, 
         for (int j = 0; j < TJ; ++j) acc[i][j] = MF::run(af[i], bf[j], acc[i][j]);
     }
     if (more) store(cur ^ 1);
+    if (more) __builtin_amdgcn_sched_group_barrier(0x020, NVM, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, NRD, 0);
+#pragma unroll
+    for (int kk = 0; kk < BK / 4 - 1; ++kk) {
+#pragma unroll
+      for (int q = 0; q < NRD; ++q) {
+        __builtin_amdgcn_sched_group_barrier(0x008, NMF / NRD > 0 ? NMF / NRD : 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < NWR; ++q) {
+        __builtin_amdgcn_sched_group_barrier(0x008, NMF / NWR > 0 ? NMF / NWR : 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+      }
+    } else {
+      __builtin_amdgcn_sched_group_barrier(0x008, NMF, 0);
+    }
+  };
+
+  const int nk = (K + BK - 1) / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  int kt = 0;
+  for (; kt + 1 < nk; ++kt) {
+    stage(kt & 1, true, (kt + 1) * BK);
     __syncthreads();
   }
+  stage(kt & 1, false, 0);
 
   // Epilogue.  acc[i][j][r] holds C[m0+wm0+16i+row(lk,r)][n0+wn0+16j+lr].
 #pragma unroll
