@@ -17,10 +17,11 @@
 // sub-tile of 32x32 MFMA tiles.  A pipeline stage moves KB bytes of k of both
 // panels from HBM through registers into double-buffered LDS (rows padded by
 // 16 bytes); one barrier per stage.  The stage body is given to the scheduler
-// as an explicit pipeline: next-stage global loads first, this stage's
-// first-step fragment reads, then MFMAs interleaved with the fragment reads
-// of the following k-step, and the last step's MFMAs interleaved with the LDS
-// writes of the next stage, so the single wave per SIMD keeps the matrix
+// as an explicit pipeline: this stage's first-step fragment reads, then MFMAs
+// interleaved with the fragment reads of the following k-step (the first
+// two k-steps also carry the next stage's global loads: measured 3-8 % faster
+// than issuing them all up front), and the last step's MFMAs interleaved with
+// the LDS writes of the next stage, so the waves of a SIMD keep the matrix
 // pipe fed.  Tiles are visited in GROUP_M-row groups and the block id is
 // remapped so that each XCD (blocks are dealt round-robin to the 8 XCDs)
 // runs a contiguous range of that order, keeping the panels a group shares
@@ -219,13 +220,14 @@ __global__ __launch_bounds__(S::THREADS) void mfma_nt_kernel(NtOperands p, Epi e
     if (more) store(cur ^ 1);
     // Pipeline order for the scheduler (masks: 0x020 VMEM read, 0x100 DS
     // read, 0x200 DS write, 0x008 MFMA).
-    if (more) __builtin_amdgcn_sched_group_barrier(0x020, NWR, 0);
     __builtin_amdgcn_sched_group_barrier(0x100, NRD, 0);
 #pragma unroll
     for (int kk = 0; kk < NKK - 1; ++kk) {
 #pragma unroll
       for (int q = 0; q < NRD; ++q) {
         __builtin_amdgcn_sched_group_barrier(0x008, NMF / NRD > 0 ? NMF / NRD : 1, 0);
+        if (more && kk < 2)  // next stage's global loads over the first two k-steps
+          __builtin_amdgcn_sched_group_barrier(0x020, (NWR + 2 * NRD - 1) / (2 * NRD), 0);
         __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
       }
     }

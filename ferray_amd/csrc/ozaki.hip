@@ -89,16 +89,16 @@ __device__ __forceinline__ int ssq_shift(Ssq a, int g) {
   return g - (a.E + ex);
 }
 
-// Row scales of A: s_i = g - e_i with ||a_i||_2 <= 2^e_i; rows holding
-// non-finite entries are flagged.  One block per row, one pass.
-__global__ __launch_bounds__(256) void oz_row_scale(const double* __restrict__ a, int K, int g,
-                                                    int* __restrict__ shift, int* __restrict__ bad) {
+// Row scale of one row of A (s_i = g - e_i with ||a_i||_2 <= 2^e_i) and
+// whether it holds a non-finite entry, computed by the whole block; every
+// thread gets the results.
+__device__ __forceinline__ void row_stats(const double* __restrict__ r, int K, int g, int& shift,
+                                          int& bad) {
   __shared__ int pe[256];
   __shared__ double ps[256];
   __shared__ int anybad;
-  const int row = blockIdx.x;
-  const double* r = a + (size_t)row * K;
   if (threadIdx.x == 0) anybad = 0;
+  __syncthreads();
   // Four independent partial sums so four loads are in flight per thread.
   Ssq acc4[4] = {{INT_MIN, 0.0}, {INT_MIN, 0.0}, {INT_MIN, 0.0}, {INT_MIN, 0.0}};
   int nonfinite = 0;
@@ -132,10 +132,8 @@ __global__ __launch_bounds__(256) void oz_row_scale(const double* __restrict__ a
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    shift[row] = ssq_shift(Ssq{pe[0], ps[0]}, g);
-    bad[row] = anybad;
-  }
+  shift = ssq_shift(Ssq{pe[0], ps[0]}, g);
+  bad = anybad;
 }
 
 // Column statistics of B (K x N row-major): partial pairs per k-slice
@@ -226,46 +224,55 @@ __device__ __forceinline__ unsigned residue_word(const Limbs (&L)[4], const OzCo
   return v;
 }
 
-// A planes: plane l is Mp x Kp int8, row-major.  Each thread covers 4
-// consecutive k of one row, lanes running along k, so the f64 loads and the
-// 4-byte residue stores of a wave are both contiguous.  vec: rows are 16-byte
-// aligned (K even, A 16-byte aligned), so the 4 values load as 2 x 16 bytes.
+// Row scale and residue planes of A in one kernel: one block per row of the
+// Mp x Kp planes.  The block first reduces the row's 2-norm (the row then
+// sits in L2), then turns it into residues, 4 consecutive k per thread, so
+// A is read from HBM once.  Rows at or beyond M are zero padding.
 template <int NM>
-__global__ __launch_bounds__(256) void oz_residues_a(const double* __restrict__ a, int M, int K,
-                                                     int Mp, int Kp, int vec,
-                                                     const int* __restrict__ shift,
-                                                     uint8_t* __restrict__ planes,
-                                                     size_t plane_bytes) {
+__global__ __launch_bounds__(256) void oz_scale_residues_a(const double* __restrict__ a, int M,
+                                                           int K, int Kp, int vec, int g,
+                                                           int* __restrict__ shift,
+                                                           int* __restrict__ bad,
+                                                           uint8_t* __restrict__ planes,
+                                                           size_t plane_bytes) {
   const OzConsts& c = g_oz[NM];
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t per_row = Kp / 4;
-  if (idx >= (size_t)Mp * per_row) return;
-  const int row = (int)(idx / per_row), k0 = (int)(idx % per_row) * 4;
-  double x[4] = {0.0, 0.0, 0.0, 0.0};
+  const int row = blockIdx.x;
+  int sh = 0;
   if (row < M) {
-    const double* src = a + (size_t)row * K + k0;
-    if (vec && k0 + 4 <= K) {
-      typedef double d2 __attribute__((ext_vector_type(2)));
-      const d2 lo = *reinterpret_cast<const d2*>(src);
-      const d2 hi = *reinterpret_cast<const d2*>(src + 2);
-      x[0] = lo.x;
-      x[1] = lo.y;
-      x[2] = hi.x;
-      x[3] = hi.y;
-    } else {
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (k0 + e < K) x[e] = src[e];
+    int bd;
+    row_stats(a + (size_t)row * K, K, g, sh, bd);
+    if (threadIdx.x == 0) {
+      shift[row] = sh;
+      bad[row] = bd;
     }
   }
-  const int sh = row < M ? shift[row] : 0;
-  Limbs L[4];
+  const double* src_row = a + (size_t)row * K;
+  uint8_t* dst_row = planes + (size_t)row * Kp;
+  for (int k0 = 4 * threadIdx.x; k0 < Kp; k0 += 4 * 256) {
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+    if (row < M) {
+      const double* src = src_row + k0;
+      if (vec && k0 + 4 <= K) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const d2 lo = *reinterpret_cast<const d2*>(src);
+        const d2 hi = *reinterpret_cast<const d2*>(src + 2);
+        x[0] = lo.x;
+        x[1] = lo.y;
+        x[2] = hi.x;
+        x[3] = hi.y;
+      } else {
 #pragma unroll
-  for (int e = 0; e < 4; ++e) L[e] = limbs_of(rint(ldexp(isfinite(x[e]) ? x[e] : 0.0, sh)));
-  uint8_t* dst = planes + (size_t)row * Kp + k0;
+        for (int e = 0; e < 4; ++e)
+          if (k0 + e < K) x[e] = src[e];
+      }
+    }
+    Limbs L[4];
 #pragma unroll
-  for (int l = 0; l < NM; ++l)
-    *reinterpret_cast<unsigned*>(dst + l * plane_bytes) = residue_word(L, c, l);
+    for (int e = 0; e < 4; ++e) L[e] = limbs_of(rint(ldexp(isfinite(x[e]) ? x[e] : 0.0, sh)));
+#pragma unroll
+    for (int l = 0; l < NM; ++l)
+      *reinterpret_cast<unsigned*>(dst_row + k0 + l * plane_bytes) = residue_word(L, c, l);
+  }
 }
 
 // B^T planes: plane l is Np x Kp int8 (row n holds column n of B).  A block
@@ -567,7 +574,6 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
   int* pe = reinterpret_cast<int*>(ws + o_cm);
   double* ps = reinterpret_cast<double*>(ws + o_cs);
 
-  hipLaunchKernelGGL(oz_row_scale, dim3((unsigned)m), dim3(256), 0, st, a, (int)k, gamma, sA, badA);
   const unsigned gn = (unsigned)cdiv(n, 256);
   hipLaunchKernelGGL(oz_init, dim3(gn), dim3(256), 0, st, badB, (int)n);
   const dim3 cgrid(gn, (unsigned)slices);
@@ -577,12 +583,11 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
                      ps, tB);
   FG_CHECK(hipGetLastError());
 
-  const size_t athreads = Mp * (Kp / 4);
   const int vec_a = (k % 2 == 0) && ((uintptr_t)a % 16 == 0);
   with_moduli(nmod, [&](auto nm) {
     constexpr int NM = decltype(nm)::value;
-    hipLaunchKernelGGL((oz_residues_a<NM>), dim3((unsigned)cdiv(athreads, 256)), dim3(256), 0, st,
-                       a, (int)m, (int)k, (int)Mp, (int)Kp, vec_a, sA, ws + o_pa, a_plane);
+    hipLaunchKernelGGL((oz_scale_residues_a<NM>), dim3((unsigned)Mp), dim3(256), 0, st, a, (int)m,
+                       (int)k, (int)Kp, vec_a, gamma, sA, badA, ws + o_pa, a_plane);
     hipLaunchKernelGGL((oz_residues_bt<NM>), dim3((unsigned)(Np / 32), (unsigned)(Kp / 128)),
                        dim3(256), 0, st, b, (int)k, (int)n, (int)Kp, tB, ws + o_pb, b_plane);
   });
