@@ -4,7 +4,7 @@
 //   i8 x i8 -> i32    one i8 pass                              (gemm_i8_signed)
 //   u8 x i8 -> i32    A's sign bit flipped on load (a - 128), plus 128*colsum(B)
 //                                                                      (gemm_i8)
-//   i16 x i16 -> i32  four i8 passes over byte planes                 (gemm_i16)
+//   i16 x i16 -> i32  four i8 byte-plane products in one launch       (gemm_i16)
 //   quantized_matmul  the u8 x i8 pass with a fused zero-point / scale epilogue
 //   bf16/f16 -> f32   one half-precision pass                 (gemm_{bf16,f16}_f32)
 //   fp8 -> f32        one pass on the 64-deep f8f6f4 instruction, OCP E4M3 or
@@ -59,6 +59,30 @@ struct EpiI32 {
     int32_t* dst = c + (size_t)row * N + col;
     if (read_c) v += (uint32_t)*dst;
     *dst = (int32_t)v;
+  }
+};
+
+// The four i16 byte-plane products of one K chunk in one launch (grid row z
+// = A plane z / 2 x B plane z % 2, shifted left by 16, 8, 8, 0): each adds
+// its shifted value into C with an atomic add.  Integer addition mod 2^32 is
+// order-independent, so the result is exact whatever order the adds land
+// in.  The first chunk's row z = 0 also adds the sum corrections.
+struct EpiI16Atomic {
+  int32_t* c;
+  int M, N;
+  const int32_t* colsum;   // 128*colsum(B) term, or null
+  const int32_t* rowsum;   // 128*rowsum(A) term, or null
+  uint32_t kconst;         // -16384*k, first chunk only
+  __device__ __forceinline__ void operator()(int row, int col, int z, int acc) const {
+    if (row >= M || col >= N) return;
+    const int shift = z == 0 ? 16 : (z == 3 ? 0 : 8);
+    uint32_t v = (uint32_t)acc << shift;
+    if (z == 0) {
+      if (colsum) v += 128u * (uint32_t)colsum[col];
+      if (rowsum) v += 128u * (uint32_t)rowsum[row];
+      v += kconst;
+    }
+    atomicAdd(reinterpret_cast<unsigned int*>(c + (size_t)row * N + col), v);
   }
 };
 
@@ -363,18 +387,21 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
         rc = FERRAY_ERR_INVALID;  // unreachable: run_nt routes k > 2^16 to run_quant_long
       }
     } else if (kind == NT_I16) {
-      // passes (A plane, B plane, shift): hi*hi<<16, hi*lo'<<8, lo'*hi<<8, lo'*lo'
-      const int pa[4] = {0, 0, 1, 1}, pb[4] = {0, 1, 0, 1}, sh[4] = {16, 8, 8, 0};
-      for (int ps = 0; ps < 4 && rc == FERRAY_OK; ++ps) {
-        NtOperands r = q;
-        r.a = q.a + (pa[ps] ? (ws + o_a1) - (ws + o_a0) : 0);
-        r.bt = q.bt + (pb[ps] ? (ws + o_bt1) - (ws + o_bt0) : 0);
-        const bool corr = first && ps == 0;
-        EpiI32 e{reinterpret_cast<int32_t*>(c), M, N, sh[ps], (ps > 0 || read_c) ? 1 : 0,
-                 corr ? csp : nullptr, corr ? rsp : nullptr,
-                 corr ? (uint32_t)(0u - 16384u * (uint32_t)k) : 0u};
-        rc = launch_mfma_nt<ELEM_I8, S, false>(r, 1, e, st);
+      // All four byte-plane products in one launch (hi*hi<<16, hi*lo'<<8,
+      // lo'*hi<<8, lo'*lo'), adding into C; C starts at zero unless the
+      // call accumulates.
+      if (first && !accumulate) {
+        hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(m * n)), dim3(256), 0, st,
+                           reinterpret_cast<uint32_t*>(c), m * n);
+        FG_CHECK(hipGetLastError());
       }
+      NtOperands r = q;
+      r.a_plane = (size_t)((ws + o_a1) - (ws + o_a0));
+      r.b_plane = (size_t)((ws + o_bt1) - (ws + o_bt0));
+      r.cross = 1;
+      EpiI16Atomic e{reinterpret_cast<int32_t*>(c), M, N, first ? csp : nullptr,
+                     first ? rsp : nullptr, first ? (uint32_t)(0u - 16384u * (uint32_t)k) : 0u};
+      rc = launch_mfma_nt<ELEM_I8, S, false>(r, 4, e, st);
     } else {
       EpiI32 e{reinterpret_cast<int32_t*>(c), M, N, 0, read_c,
                (kind == NT_U8 && first) ? csp : nullptr, nullptr, 0u};
