@@ -301,4 +301,9 @@ __device__ __forceinline__ bool real_tile_interior(const RealArgs<T>& p, int m0,
          (p.N * sizeof(T)) % 16 == 0 && ((uintptr_t)p.a % 16) == 0 && ((uintptr_t)p.b % 16) == 0;
 }
 
+// Host launcher of the 64 x 64 tile kernel for products too small to fill
+// the chip with 128 x 128 tiles (gemm_small.hip).
+template <typename T, bool CPLX>
+int launch_real_small(const RealArgs<T>& p, hipStream_t st);
+
 }  // namespace fg
