@@ -40,22 +40,17 @@ constexpr size_t NT_K_CHUNK = 1u << 16;
 // Epilogues
 // ---------------------------------------------------------------------
 
-// i32 output.  v = (acc << shift) + 128*colsum[col] (if colsum) + corr[row]
-// (if rowcorr) + kconst + (old C if read_c).
+// i32 output of the i8 products: v = acc + 128*colsum[col] (if colsum; u8 A
+// was shifted by -128) + (old C if read_c), all wrapping.
 struct EpiI32 {
   int32_t* c;
   int M, N;
-  int shift;
   int read_c;
-  const int32_t* colsum;   // u8: 128*colsum(B); i16: 128*colsum(B)
-  const int32_t* rowsum;   // i16: 128*rowsum(A)
-  uint32_t kconst;         // i16: -16384*k
+  const int32_t* colsum;   // u8: 128*colsum(B)
   __device__ __forceinline__ void operator()(int row, int col, int, int acc) const {
     if (row >= M || col >= N) return;
-    uint32_t v = (uint32_t)acc << shift;
+    uint32_t v = (uint32_t)acc;
     if (colsum) v += 128u * (uint32_t)colsum[col];
-    if (rowsum) v += 128u * (uint32_t)rowsum[row];
-    v += kconst;
     int32_t* dst = c + (size_t)row * N + col;
     if (read_c) v += (uint32_t)*dst;
     *dst = (int32_t)v;
@@ -403,8 +398,8 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
                      first ? rsp : nullptr, first ? (uint32_t)(0u - 16384u * (uint32_t)k) : 0u};
       rc = launch_mfma_nt<ELEM_I8, S, false>(r, 4, e, st);
     } else {
-      EpiI32 e{reinterpret_cast<int32_t*>(c), M, N, 0, read_c,
-               (kind == NT_U8 && first) ? csp : nullptr, nullptr, 0u};
+      EpiI32 e{reinterpret_cast<int32_t*>(c), M, N, read_c,
+               (kind == NT_U8 && first) ? csp : nullptr};
       rc = xor_a ? launch_mfma_nt<ELEM_I8, S, true>(q, 1, e, st)
                  : launch_mfma_nt<ELEM_I8, S, false>(q, 1, e, st);
     }
