@@ -352,7 +352,7 @@ def serving(fa, count: int = 20000):
            "mix": "f64 0.5, f32 0.3, c64 0.1, i8 0.1", "p99_target_ms": SERVE_P99_TARGET_MS,
            "lanes": 4, "max_batch": 256}
     rates = {"grouped": [2e4, 5e4, 7.5e4, 1e5, 1.25e5, 1.5e5, 2e5],
-             "per_request": [1e4, 1.5e4, 2e4, 2.25e4, 2.5e4, 3e4]}
+             "per_request": [1e4, 2e4, 3e4, 3.5e4, 4e4, 5e4, 6e4]}
     best = {}
     for name, mode in (("grouped", sv.SERVE_GROUPED), ("per_request", sv.SERVE_PER_REQUEST)):
         rows = []
