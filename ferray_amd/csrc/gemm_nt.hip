@@ -63,6 +63,7 @@ struct EpiI32 {
 // order-independent, so the result is exact whatever order the adds land
 // in.  The first chunk's row z = 0 also adds the sum corrections.
 struct EpiI16Atomic {
+  static constexpr bool staged_loads = true;
   int32_t* c;
   int M, N;
   const int32_t* colsum;   // 128*colsum(B) term, or null
