@@ -94,10 +94,13 @@ struct EpiQuant {
   uint32_t azp, bzp, kab;
   float scale;
   __device__ __forceinline__ void operator()(int row, int col, int, int acc) const {
+    // The sums are read unconditionally (clamped indices, always in bounds)
+    // so the compiler can issue every element's loads ahead of the stores.
+    const uint32_t cs = (uint32_t)colsum[min(col, N - 1)];
+    const uint32_t rs = (uint32_t)rowsum[min(row, M - 1)];
     if (row >= M || col >= N) return;
-    const uint32_t cs = (uint32_t)colsum[col];
     const uint32_t raw = (uint32_t)acc + 128u * cs;
-    const uint32_t v = raw - azp * cs - bzp * (uint32_t)rowsum[row] + kab;
+    const uint32_t v = raw - azp * cs - bzp * rs + kab;
     float out = scale * (float)(int32_t)v;
     float* dst = c + (size_t)row * N + col;
     if (accumulate) out = *dst + out;
@@ -211,6 +214,36 @@ __global__ void pack_a_kernel(const E* __restrict__ a, int M, int K, int Mp, int
   }
 }
 
+// u8 -> u8 - 128 (sign bit flipped) over a tile-aligned A, one block per row
+// of k16 16-byte words; with `rowsum` also the wrapping sum of the row's u8
+// values (the same sums rowsum_kernel gives).
+__global__ void flip_sign_kernel(const uint4* __restrict__ a, uint4* __restrict__ out, int k16,
+                                 int32_t* __restrict__ rowsum) {
+  __shared__ uint32_t part[256];
+  const size_t base = (size_t)blockIdx.x * k16;
+  uint32_t s = 0;
+  for (int i = threadIdx.x; i < k16; i += blockDim.x) {
+    uint4 v = a[base + i];
+    s = __builtin_amdgcn_udot4(v.x, 0x01010101u, s, false);
+    s = __builtin_amdgcn_udot4(v.y, 0x01010101u, s, false);
+    s = __builtin_amdgcn_udot4(v.z, 0x01010101u, s, false);
+    s = __builtin_amdgcn_udot4(v.w, 0x01010101u, s, false);
+    v.x ^= 0x80808080u;
+    v.y ^= 0x80808080u;
+    v.z ^= 0x80808080u;
+    v.w ^= 0x80808080u;
+    out[base + i] = v;
+  }
+  if (!rowsum) return;
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rowsum[blockIdx.x] = (int32_t)part[0];
+}
+
 // Wrapping row sums of a row-major M x K matrix, one block per row.
 template <typename E>
 __global__ void rowsum_kernel(const E* __restrict__ a, int K, int32_t* out) {
@@ -270,15 +303,20 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
   const size_t Mp = align_up(m, S::BM), Np = align_up(n, S::BN);
   const bool need_colsum = (kind == NT_U8 || kind == NT_QUANT || kind == NT_I16);
   const bool need_rowsum = (kind == NT_QUANT || kind == NT_I16);
-  const bool direct_a = (kind != NT_I16) && (m % S::BM == 0) && ((k * es) % S::KB == 0) &&
-                        ((uintptr_t)a % 16 == 0);
+  const bool aligned_a = (kind != NT_I16) && (m % S::BM == 0) && ((k * es) % S::KB == 0) &&
+                         ((uintptr_t)a % 16 == 0);
+  // Tile-aligned A is read in place, except u8 A on the 256 x 256 shape:
+  // there a flipped copy lets the product run the direct-to-LDS kernel,
+  // whose copies move bytes unmodified.
+  const bool flip_a = aligned_a && is_u8 && S::BM == 256;
+  const bool direct_a = aligned_a && !flip_a;
 
   Carver cv;
   const size_t o_bt0 = cv.take(Np * Kp * pe);
   const size_t o_bt1 = planes == 2 ? cv.take(Np * Kp) : 0;
   const size_t o_cs = cv.take(Np * 4);
   const size_t o_rs = cv.take(Mp * 4);
-  const size_t o_a0 = direct_a ? 0 : cv.take(Mp * Kp * pe);
+  const size_t o_a0 = direct_a ? 0 : cv.take(flip_a ? m * k : Mp * Kp * pe);
   const size_t o_a1 = (!direct_a && planes == 2) ? cv.take(Mp * Kp) : 0;
   void* wsv = nullptr;
   int s = workspace(st, cv.off, &wsv);
@@ -306,7 +344,7 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
                        ws + o_bt0, nullptr, csp);
   FG_CHECK(hipGetLastError());
 
-  if (need_rowsum) {
+  if (need_rowsum && !flip_a) {
     if (kind == NT_I16)
       hipLaunchKernelGGL(rowsum_kernel<int16_t>, dim3((unsigned)m), dim3(256), 0, st,
                          reinterpret_cast<const int16_t*>(a), (int)k, rsp);
@@ -328,6 +366,13 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
     p.a = reinterpret_cast<const uint8_t*>(a);
     p.lda = k * es;
     xor_a = is_u8;
+  } else if (flip_a) {  // flipped copy (and the row sums) in one pass over A
+    hipLaunchKernelGGL(flip_sign_kernel, dim3((unsigned)m), dim3(256), 0, st,
+                       reinterpret_cast<const uint4*>(a), reinterpret_cast<uint4*>(ws + o_a0),
+                       (int)(k / 16), rsp);
+    FG_CHECK(hipGetLastError());
+    p.a = ws + o_a0;
+    p.lda = k;
   } else {
     const unsigned g = grid_for(Mp * Kp);
     if (kind == NT_I16)

@@ -343,7 +343,7 @@ struct NtStaged<E, std::void_t<decltype(E::staged_loads)>> : std::bool_constant<
 typedef __attribute__((address_space(1))) void nt_gvoid;
 typedef __attribute__((address_space(3))) void nt_lvoid;
 
-template <int ELEM, bool XOR_A, class Epi>
+template <int ELEM, class Epi>
 __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi) {
   using Acc = NtDmaAcc<ELEM>;
   using acc_t = typename Acc::type;
@@ -411,10 +411,7 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
   };
   auto read_step = [&](const uint8_t* s, int kk, frag* fa, frag* fb) __attribute__((always_inline)) {
 #pragma unroll
-    for (int i = 0; i < TI; ++i) {
-      fa[i] = read(s + arow + i * TS * KB, kk);
-      if (XOR_A) fa[i] ^= (frag)(0x80808080u);
-    }
+    for (int i = 0; i < TI; ++i) fa[i] = read(s + arow + i * TS * KB, kk);
 #pragma unroll
     for (int j = 0; j < TJ; ++j) fb[j] = read(s + brow + j * TS * KB, kk);
   };
@@ -524,8 +521,8 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
 // Launches the NT main loop over `planes` independent plane pairs.  The
 // 256 x 256 shape runs the direct-to-LDS kernel when 256 rows of either
 // operand span less than 4 GiB (its in-tile offsets are 32-bit), A needs no
-// sign flip (the flip's extra fragment registers spill there) and the
-// epilogue does not ask for staged loads.
+// sign flip (the copies move bytes unmodified; flipping fragments after the
+// reads spills registers) and the epilogue does not ask for staged loads.
 template <int ELEM, class S, bool XOR_A, class Epi>
 inline int launch_mfma_nt(const NtOperands& p, int planes, const Epi& epi, hipStream_t st) {
   const size_t tiles = (size_t)p.tiles_m * p.tiles_n;
@@ -533,20 +530,20 @@ inline int launch_mfma_nt(const NtOperands& p, int planes, const Epi& epi, hipSt
   if (tiles > 0x7fffffff || planes > 65535) return FERRAY_ERR_INVALID;
   if (p.kbytes <= 0 || p.kbytes % S::KB) return FERRAY_ERR_INVALID;
   const dim3 grid((unsigned)tiles, (unsigned)planes);
-  constexpr bool BIG = S::BM == 256 && S::BN == 256 && S::KB == 128;
-  if (BIG && !XOR_A && !NtStaged<Epi>::value &&
-      256 * (p.lda > p.ldb ? p.lda : p.ldb) < ((size_t)1 << 32)) {
-    auto kern = mfma_nt_dma_kernel<ELEM, XOR_A, Epi>;
-    constexpr int lds = 2 * 512 * 128;
-    static bool attr_set = false;  // every caller sets the same value; a race is benign
-    if (!attr_set) {
-      FG_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   lds));
-      attr_set = true;
+  if constexpr (S::BM == 256 && S::BN == 256 && S::KB == 128 && !XOR_A && !NtStaged<Epi>::value) {
+    if (256 * (p.lda > p.ldb ? p.lda : p.ldb) < ((size_t)1 << 32)) {
+      auto kern = mfma_nt_dma_kernel<ELEM, Epi>;
+      constexpr int lds = 2 * 512 * 128;
+      static bool attr_set = false;  // every caller sets the same value; a race is benign
+      if (!attr_set) {
+        FG_CHECK(hipFuncSetAttribute((const void*)kern,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        attr_set = true;
+      }
+      hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, p, epi);
+      FG_CHECK(hipGetLastError());
+      return FERRAY_OK;
     }
-    hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, p, epi);
-    FG_CHECK(hipGetLastError());
-    return FERRAY_OK;
   }
   auto kern = mfma_nt_kernel<ELEM, S, XOR_A, Epi>;
   static bool attr_set = false;
