@@ -281,13 +281,9 @@ __global__ __launch_bounds__(S::THREADS) void mfma_nt_kernel(NtOperands p, Epi e
 // no LDS write instructions.  One such instruction writes 1 KiB of LDS at
 // consecutive 16-byte slots (lane l at slot l), so the wave fills 8 rows of
 // 128 bytes and each lane chooses which 16-byte chunk of its row it fetches:
-// row r keeps chunk c at slot c ^ swz(r),
-//   swz(r) = (r & 7) ^ ((r >> 1) & 1) ^ ((r >> 3) & 1).
-// A fragment read has lanes on 16 (or 32) consecutive rows, one or two
-// chunks; the swizzle keeps every 8-lane group, and every pair of 8-lane
-// groups the LDS serves together, on distinct banks for both the 16 x 16
-// and the 32 x 32 read patterns (the plain (r & 7) ^ ((r >> 3) & 1) left the
-// 16 x 16 reads 2-way conflicted: rocprofv3 SQ_LDS_BANK_CONFLICT).
+// row r keeps chunk c at slot c ^ swz(r), swz(r) = (r & 7) ^ ((r >> 3) & 1).
+// Fragment reads (16 rows of one chunk per quarter wave, or 8 rows per
+// eighth) then hit distinct banks.
 //
 // The pipeline is rotated: a stage's last k-step is multiplied after the
 // barrier that opens the next stage, interleaved with that stage's copy
@@ -375,7 +371,7 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
   // panel: lane l writes slot l & 7 of row l >> 3.  Offsets within the tile
   // are 32-bit (the launcher checks 256 rows fit).
   const int lrow = lane >> 3, lslot = lane & 7;
-  const int gchunk = lslot ^ lrow ^ ((lrow >> 1) & 1) ^ (wave & 1);  // swz of row 8 (4q + w) + lrow
+  const int gchunk = lslot ^ lrow ^ (wave & 1);  // swz of row 8 (4q + w) + lrow
   const unsigned a_off = (unsigned)((wave * 8 + lrow) * p.lda + gchunk * 16);
   const unsigned b_off = (unsigned)((wave * 8 + lrow) * p.ldb + gchunk * 16);
   const unsigned a_q = (unsigned)(32 * p.lda), b_q = (unsigned)(32 * p.ldb);
@@ -397,7 +393,7 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
   // Fragment of k-step kk: lane l reads row l % TS of the tile, 16-byte
   // chunk(s) kk * CH + (TS == 32 ? h * 2 + l / 32 : l / 16) (h = second read
   // of an fp8 fragment), at its swizzled slot.
-  const int swz = (lr & 7) ^ ((lr >> 1) & 1) ^ ((lr >> 3) & 1);
+  const int swz = (lr & 7) ^ ((lr >> 3) & 1);
   int xo[NKK][RD];
 #pragma unroll
   for (int kk = 0; kk < NKK; ++kk)
