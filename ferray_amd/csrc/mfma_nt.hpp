@@ -281,9 +281,11 @@ __global__ __launch_bounds__(S::THREADS) void mfma_nt_kernel(NtOperands p, Epi e
 // no LDS write instructions.  One such instruction writes 1 KiB of LDS at
 // consecutive 16-byte slots (lane l at slot l), so the wave fills 8 rows of
 // 128 bytes and each lane chooses which 16-byte chunk of its row it fetches:
-// row r keeps chunk c at slot c ^ swz(r), swz(r) = (r & 7) ^ ((r >> 3) & 1).
-// Fragment reads (16 rows of one chunk per quarter wave, or 8 rows per
-// eighth) then hit distinct banks.
+// row r keeps chunk c at slot c ^ swz(r), swz(r) = (r & 7) ^ ((r >> 3) & 1),
+// so 16 consecutive rows of one chunk land on distinct banks.  rocprofv3
+// counts no LDS bank conflicts for the 32 x 32 reads (fp8); the 16 x 16
+// reads show 4 conflict cycles per read with this and with the other
+// swizzles tried, with no measurable effect on the time.
 //
 // The pipeline is rotated: a stage's last k-step is multiplied after the
 // barrier that opens the next stage, interleaved with that stage's copy
