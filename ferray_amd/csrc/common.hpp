@@ -42,6 +42,14 @@ void release_workspace(hipStream_t stream);
 // Runtime initialisation on the calling thread (device selection).
 int ensure_device();
 
+// Pageable host memory <-> device, enqueued on `stream`.  Copies of 8 MiB
+// and more go through the calling thread's pinned bounce buffers; the upload
+// returns once the host buffer may be reused, the download once the host
+// buffer holds the data (smaller downloads only enqueue: synchronise the
+// stream before reading them).
+int host_upload(void* dst, const void* src, size_t bytes, hipStream_t stream);
+int host_download(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
 inline size_t cdiv(size_t a, size_t b) { return (a + b - 1) / b; }
 
 }  // namespace fg

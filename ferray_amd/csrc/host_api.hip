@@ -2,8 +2,21 @@
 // staged into per-thread device buffers, the device entry point runs on the
 // thread's stream, and the call returns once C is back in host memory.
 // Any failure returns false with the reason in ferray_gemm_last_status().
+//
+// Large operands (8 MiB and up) move through pinned bounce buffers: a ring
+// of four 16 MiB pinned slots per thread, filled and drained by a small pool
+// of host threads while the DMA engine moves the neighbouring chunk.  The
+// caller's buffers are ordinary pageable memory, which the DMA engine reads
+// and writes at roughly a third of the pinned rate.
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
 
 #include "common.hpp"
 
@@ -36,6 +49,166 @@ struct Staging {
 };
 thread_local Staging t_staging;
 
+// ---------------------------------------------------------------------------
+// Pinned bounce buffers
+// ---------------------------------------------------------------------------
+
+constexpr size_t kChunk = (size_t)16 << 20;        // bytes per pinned slot
+constexpr int kSlots = 4;                          // slots per thread
+constexpr size_t kPipelineMin = (size_t)8 << 20;   // smaller copies go direct
+
+// Process-wide helper threads for large host memcpy.  copy() splits a range
+// into pieces of at least 1 MiB, runs one piece on the caller and the rest
+// on the helpers, and returns when all are done.  Concurrent callers share
+// the queue.  The pool is never destroyed (no joins at process exit).
+class CopyPool {
+ public:
+  static CopyPool& get() {
+    static CopyPool* pool = new CopyPool();
+    return *pool;
+  }
+  void copy(void* dst, const void* src, size_t n) {
+    const size_t pieces = std::min<size_t>(workers_ + 1, std::max<size_t>(1, n >> 20));
+    if (pieces <= 1) {
+      std::memcpy(dst, src, n);
+      return;
+    }
+    const size_t step = (n / pieces + 4095) & ~(size_t)4095;
+    std::atomic<size_t> left{0};
+    size_t off = step;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      for (; off < n; off += step) {
+        q_.push_back(Job{(char*)dst + off, (const char*)src + off, std::min(step, n - off), &left});
+        left.fetch_add(1, std::memory_order_relaxed);
+      }
+    }
+    cv_.notify_all();
+    std::memcpy(dst, src, std::min(step, n));
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [&] { return left.load(std::memory_order_acquire) == 0; });
+  }
+
+ private:
+  struct Job {
+    char* dst;
+    const char* src;
+    size_t n;
+    std::atomic<size_t>* left;
+  };
+  CopyPool() {
+    const char* env = std::getenv("FERRAY_HOST_COPY_THREADS");
+    unsigned hw = std::thread::hardware_concurrency();
+    long want = env ? std::atol(env) : (long)std::min(8u, hw ? hw : 1u);
+    workers_ = (size_t)std::max(0L, std::min(want, 64L) - 1);
+    for (size_t i = 0; i < workers_; ++i) std::thread([this] { run(); }).detach();
+  }
+  void run() {
+    for (;;) {
+      Job j;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return !q_.empty(); });
+        j = q_.front();
+        q_.pop_front();
+      }
+      std::memcpy(j.dst, j.src, j.n);
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        j.left->fetch_sub(1, std::memory_order_release);
+      }
+      done_.notify_all();
+    }
+  }
+  size_t workers_ = 0;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  std::deque<Job> q_;
+};
+
+// A thread's ring of pinned slots.  ev[i] is recorded after the last DMA
+// that touches slot i, so host code waits on it before reusing the slot.
+struct PinnedRing {
+  void* slot[kSlots] = {};
+  hipEvent_t ev[kSlots] = {};
+  unsigned next = 0;
+  bool ready = false;
+  ~PinnedRing() {
+    for (int i = 0; i < kSlots; ++i) {
+      if (ev[i]) (void)hipEventDestroy(ev[i]);
+      if (slot[i]) (void)hipHostFree(slot[i]);
+    }
+  }
+  int init() {
+    if (ready) return FERRAY_OK;
+    for (int i = 0; i < kSlots; ++i) {
+      if (!slot[i]) FG_CHECK(hipHostMalloc(&slot[i], kChunk, hipHostMallocDefault));
+      if (!ev[i]) FG_CHECK(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+    }
+    ready = true;
+    return FERRAY_OK;
+  }
+};
+thread_local PinnedRing t_ring;
+
+}  // namespace
+
+// Host -> device, enqueued on `st`; the host buffer may be reused on return.
+int host_upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes < kPipelineMin) {
+    FG_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    return FERRAY_OK;
+  }
+  int s = t_ring.init();
+  if (s) return s;
+  CopyPool& pool = CopyPool::get();
+  for (size_t off = 0; off < bytes; off += kChunk) {
+    const int sl = (int)(t_ring.next++ % kSlots);
+    const size_t len = std::min(kChunk, bytes - off);
+    FG_CHECK(hipEventSynchronize(t_ring.ev[sl]));  // the slot's previous DMA is done
+    pool.copy(t_ring.slot[sl], (const char*)src + off, len);
+    FG_CHECK(hipMemcpyAsync((char*)dst + off, t_ring.slot[sl], len, hipMemcpyHostToDevice, st));
+    FG_CHECK(hipEventRecord(t_ring.ev[sl], st));
+  }
+  return FERRAY_OK;
+}
+
+// Device -> host after the work already on `st`; returns once dst holds the
+// data.  Small copies are enqueued only (the caller synchronises).
+int host_download(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes < kPipelineMin) {
+    FG_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    return FERRAY_OK;
+  }
+  int s = t_ring.init();
+  if (s) return s;
+  CopyPool& pool = CopyPool::get();
+  const size_t n = (bytes + kChunk - 1) / kChunk;
+  const unsigned base = t_ring.next;
+  t_ring.next += (unsigned)n;
+  auto enqueue = [&](size_t i) -> int {
+    const int sl = (int)((base + i) % kSlots);
+    const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
+    FG_CHECK(hipEventSynchronize(t_ring.ev[sl]));  // prior use, possibly on another stream
+    FG_CHECK(hipMemcpyAsync(t_ring.slot[sl], (const char*)src + off, len, hipMemcpyDeviceToHost,
+                            st));
+    FG_CHECK(hipEventRecord(t_ring.ev[sl], st));
+    return FERRAY_OK;
+  };
+  for (size_t i = 0; i < n && i < (size_t)kSlots; ++i)
+    if ((s = enqueue(i))) return s;
+  for (size_t i = 0; i < n; ++i) {
+    const int sl = (int)((base + i) % kSlots);
+    const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
+    FG_CHECK(hipEventSynchronize(t_ring.ev[sl]));
+    pool.copy((char*)dst + off, t_ring.slot[sl], len);
+    if (i + kSlots < n && (s = enqueue(i + kSlots))) return s;
+  }
+  return FERRAY_OK;
+}
+
+namespace {
+
 // Runs `launch(dA, dB, dC, stream)` with A (abytes), B (bbytes) uploaded and
 // C (cbytes) uploaded first when `c_in` and downloaded afterwards.
 template <typename F>
@@ -49,15 +222,15 @@ int staged(const void* a, size_t abytes, const void* b, size_t bbytes, void* c, 
   if ((s = t_staging.get(0, abytes, &da))) return s;
   if ((s = t_staging.get(1, bbytes, &db))) return s;
   if ((s = t_staging.get(2, cbytes, &dc))) return s;
-  if (abytes) FG_CHECK(hipMemcpyAsync(da, a, abytes, hipMemcpyHostToDevice, st));
-  if (bbytes) FG_CHECK(hipMemcpyAsync(db, b, bbytes, hipMemcpyHostToDevice, st));
-  if (c_in && cbytes) FG_CHECK(hipMemcpyAsync(dc, c, cbytes, hipMemcpyHostToDevice, st));
-  s = launch(da, db, dc, st);
-  if (s) {
+  auto fail = [&](int e) {
     (void)hipStreamSynchronize(st);
-    return s;
-  }
-  if (cbytes) FG_CHECK(hipMemcpyAsync(c, dc, cbytes, hipMemcpyDeviceToHost, st));
+    return e;
+  };
+  if (abytes && (s = host_upload(da, a, abytes, st))) return fail(s);
+  if (bbytes && (s = host_upload(db, b, bbytes, st))) return fail(s);
+  if (c_in && cbytes && (s = host_upload(dc, c, cbytes, st))) return fail(s);
+  if ((s = launch(da, db, dc, st))) return fail(s);
+  if (cbytes && (s = host_download(c, dc, cbytes, st))) return fail(s);
   FG_CHECK(hipStreamSynchronize(st));
   return FERRAY_OK;
 }

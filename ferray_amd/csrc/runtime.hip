@@ -218,7 +218,8 @@ int ferray_dev_upload(void* dst, const void* src, size_t bytes, void* stream) {
   if (s) return s;
   if (!bytes) return FERRAY_OK;
   hipStream_t st = resolve_stream(stream);
-  FG_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+  if (!st) return FERRAY_ERR_HIP;
+  if ((s = host_upload(dst, src, bytes, st))) return s;
   FG_CHECK(hipStreamSynchronize(st));
   return FERRAY_OK;
 }
@@ -228,7 +229,8 @@ int ferray_dev_download(void* dst, const void* src, size_t bytes, void* stream) 
   if (s) return s;
   if (!bytes) return FERRAY_OK;
   hipStream_t st = resolve_stream(stream);
-  FG_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+  if (!st) return FERRAY_ERR_HIP;
+  if ((s = host_download(dst, src, bytes, st))) return s;
   FG_CHECK(hipStreamSynchronize(st));
   return FERRAY_OK;
 }
