@@ -32,10 +32,15 @@ inline int map_hip_error(hipError_t e) {
 // library stream (created on first use, default priority, non-blocking).
 hipStream_t resolve_stream(void* stream);
 
-// Grow-only device scratch owned by the library, one buffer per stream.
-// Work ordered on `stream` may use the returned memory until the next
-// call for the same stream; growing synchronises that stream first.
-int workspace(hipStream_t stream, size_t bytes, void** out);
+// Grow-only device scratch owned by the library, WS_SLOTS buffers per
+// stream.  Work ordered on `stream` may use the returned memory until the
+// next call for the same stream and slot; growing synchronises that stream
+// first.  Slot 0 serves the packing and split passes (and TRSM, which keeps
+// slot-0 pointers while it calls the real GEMM); slot WS_PAD is used only by
+// the real GEMM's padded copies, so a caller holding slot-0 memory may call
+// it.
+constexpr int WS_SLOTS = 2, WS_PAD = 1;
+int workspace(hipStream_t stream, size_t bytes, void** out, int slot = 0);
 // Frees the workspace of a stream that is about to be destroyed.
 void release_workspace(hipStream_t stream);
 

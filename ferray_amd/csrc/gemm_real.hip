@@ -26,6 +26,9 @@
 // remapped so each XCD (blocks are dealt round-robin to the 8 XCDs) works on
 // a contiguous range of that order, which keeps the A and B panels a group
 // shares inside one XCD's L2.
+#include <algorithm>
+#include <type_traits>
+
 #include "real_tile.hpp"
 
 namespace fg {
@@ -341,6 +344,44 @@ __global__ void scale_c_kernel(T* c, size_t count, T beta_re, T beta_im) {
   }
 }
 
+// One complex element as stored (re, im); aligned as T, so unaligned
+// complex operands are read correctly.
+template <typename T>
+struct CplxPair {
+  T re, im;
+};
+
+// Zero-padded copy: dst (rows_p x cols_p, row pitch cols_p) takes src
+// (rows x cols, row pitch cols) in its top-left corner and zeros elsewhere.
+// Elements are E (the real or complex scalar).
+template <typename E>
+__global__ void pad_copy_kernel(E* __restrict__ dst, const E* __restrict__ src, size_t rows,
+                                size_t cols, size_t rows_p, size_t cols_p) {
+  const size_t total = rows_p * cols_p;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = i / cols_p, c = i - r * cols_p;
+    dst[i] = (r < rows && c < cols) ? src[r * cols + c] : E{};
+  }
+}
+
+// dst (rows x cols, pitch cols) = top-left corner of src (pitch cols_p).
+template <typename E>
+__global__ void unpad_copy_kernel(E* __restrict__ dst, const E* __restrict__ src, size_t rows,
+                                  size_t cols, size_t cols_p) {
+  const size_t total = rows * cols;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = i / cols, c = i - r * cols;
+    dst[i] = src[r * cols_p + c];
+  }
+}
+
+template <typename E>
+static unsigned copy_grid(size_t count) {
+  return (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(count, 256), 65536));
+}
+
 template <typename T, bool CPLX>
 static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, const T* b,
                        T br, T bi, T* c, hipStream_t st) {
@@ -379,12 +420,56 @@ static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, con
   const bool aligned = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0);
   const bool fast = aligned && m % RB_M == 0 && (n * mult) % RB_N == 0 &&
                     (k * mult) % StageK<T>::value == 0;
-  if (fast)
+  if (fast) {
     hipLaunchKernelGGL((gemm_real_kernel<T, CPLX, false>), dim3((unsigned)tiles), dim3(R_THREADS),
                        0, st, p);
-  else
-    hipLaunchKernelGGL((gemm_real_kernel<T, CPLX, true>), dim3((unsigned)tiles), dim3(R_THREADS),
-                       0, st, p);
+    FG_CHECK(hipGetLastError());
+    return FERRAY_OK;
+  }
+  // Ragged or unaligned: copy A, B (and C when it is read) into zero-padded
+  // buffers of whole tiles, run the unguarded kernel there and copy C's
+  // corner back.  The padding adds only zero products after each element's
+  // own k-sequence, so the results equal the guarded kernel's bit for bit;
+  // the guarded kernel remains for shapes where padding would add more than
+  // 15 % work or the workspace cannot be had.
+  using E = typename std::conditional<CPLX, CplxPair<T>, T>::type;
+  const size_t Mp = cdiv(m, RB_M) * RB_M;
+  const size_t Np = cdiv(n * mult, RB_N) * RB_N / mult;
+  const size_t Kp = cdiv(k * mult, StageK<T>::value) * StageK<T>::value / mult;
+  const double extra = (double)Mp * Np * Kp / ((double)m * n * k);
+  void* ws = nullptr;
+  if (extra <= 1.15 &&
+      workspace(st, (Mp * Kp + Kp * Np + Mp * Np) * sizeof(E) + 3 * 256, &ws, WS_PAD) ==
+          FERRAY_OK) {
+    E* ap = reinterpret_cast<E*>(ws);
+    E* bp = reinterpret_cast<E*>((uint8_t*)ap + cdiv(Mp * Kp * sizeof(E), 256) * 256);
+    E* cp = reinterpret_cast<E*>((uint8_t*)bp + cdiv(Kp * Np * sizeof(E), 256) * 256);
+    hipLaunchKernelGGL(pad_copy_kernel<E>, dim3(copy_grid<E>(Mp * Kp)), dim3(256), 0, st, ap,
+                       reinterpret_cast<const E*>(a), m, k, Mp, Kp);
+    hipLaunchKernelGGL(pad_copy_kernel<E>, dim3(copy_grid<E>(Kp * Np)), dim3(256), 0, st, bp,
+                       reinterpret_cast<const E*>(b), k, n, Kp, Np);
+    if (!p.beta_zero)
+      hipLaunchKernelGGL(pad_copy_kernel<E>, dim3(copy_grid<E>(Mp * Np)), dim3(256), 0, st, cp,
+                         reinterpret_cast<const E*>(c), m, n, Mp, Np);
+    FG_CHECK(hipGetLastError());
+    RealArgs<T> q = p;
+    q.a = reinterpret_cast<const T*>(ap);
+    q.b = reinterpret_cast<const T*>(bp);
+    q.c = reinterpret_cast<T*>(cp);
+    q.M = (int)Mp;
+    q.N = (int)(Np * mult);
+    q.K = (int)(Kp * mult);
+    const size_t ptiles = (Mp / RB_M) * (Np * mult / RB_N);
+    hipLaunchKernelGGL((gemm_real_kernel<T, CPLX, false>), dim3((unsigned)ptiles),
+                       dim3(R_THREADS), 0, st, q);
+    hipLaunchKernelGGL(unpad_copy_kernel<E>, dim3(copy_grid<E>(m * n)), dim3(256), 0, st,
+                       reinterpret_cast<E*>(c), cp, m, n, Np);
+    FG_CHECK(hipGetLastError());
+    return FERRAY_OK;
+  }
+  (void)hipGetLastError();  // a failed workspace allocation is not this call's error
+  hipLaunchKernelGGL((gemm_real_kernel<T, CPLX, true>), dim3((unsigned)tiles), dim3(R_THREADS),
+                     0, st, p);
   FG_CHECK(hipGetLastError());
   return FERRAY_OK;
 }

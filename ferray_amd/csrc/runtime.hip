@@ -32,8 +32,11 @@ struct Workspace {
   void* ptr = nullptr;
   size_t bytes = 0;
 };
+struct StreamWorkspaces {
+  Workspace slot[WS_SLOTS];
+};
 std::mutex g_ws_mu;
-std::unordered_map<hipStream_t, Workspace> g_ws;
+std::unordered_map<hipStream_t, StreamWorkspaces> g_ws;
 
 void init_once() {
   int count = 0;
@@ -80,16 +83,20 @@ void release_workspace(hipStream_t stream) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
   auto it = g_ws.find(stream);
   if (it == g_ws.end()) return;
-  if (it->second.ptr) {
-    (void)hipStreamSynchronize(stream);
-    (void)hipFree(it->second.ptr);
+  bool synced = false;
+  for (Workspace& w : it->second.slot) {
+    if (!w.ptr) continue;
+    if (!synced) (void)hipStreamSynchronize(stream);
+    synced = true;
+    (void)hipFree(w.ptr);
   }
   g_ws.erase(it);
 }
 
-int workspace(hipStream_t stream, size_t bytes, void** out) {
+int workspace(hipStream_t stream, size_t bytes, void** out, int slot) {
+  if (slot < 0 || slot >= WS_SLOTS) return FERRAY_ERR_INVALID;
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  Workspace& w = g_ws[stream];
+  Workspace& w = g_ws[stream].slot[slot];
   if (w.bytes < bytes) {
     if (w.ptr) {
       FG_CHECK(hipStreamSynchronize(stream));
