@@ -353,33 +353,28 @@ struct CplxPair {
 
 // Zero-padded copy: dst (rows_p x cols_p, row pitch cols_p) takes src
 // (rows x cols, row pitch cols) in its top-left corner and zeros elsewhere.
-// Elements are E (the real or complex scalar).
+// Elements are E (the real or complex scalar); x covers columns, y strides
+// over rows.
 template <typename E>
 __global__ void pad_copy_kernel(E* __restrict__ dst, const E* __restrict__ src, size_t rows,
                                 size_t cols, size_t rows_p, size_t cols_p) {
-  const size_t total = rows_p * cols_p;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const size_t r = i / cols_p, c = i - r * cols_p;
-    dst[i] = (r < rows && c < cols) ? src[r * cols + c] : E{};
-  }
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols_p) return;
+  for (size_t r = blockIdx.y; r < rows_p; r += gridDim.y)
+    dst[r * cols_p + c] = (r < rows && c < cols) ? src[r * cols + c] : E{};
 }
 
 // dst (rows x cols, pitch cols) = top-left corner of src (pitch cols_p).
 template <typename E>
 __global__ void unpad_copy_kernel(E* __restrict__ dst, const E* __restrict__ src, size_t rows,
                                   size_t cols, size_t cols_p) {
-  const size_t total = rows * cols;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const size_t r = i / cols, c = i - r * cols;
-    dst[i] = src[r * cols_p + c];
-  }
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  for (size_t r = blockIdx.y; r < rows; r += gridDim.y) dst[r * cols + c] = src[r * cols_p + c];
 }
 
-template <typename E>
-static unsigned copy_grid(size_t count) {
-  return (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(count, 256), 65536));
+static dim3 copy_grid(size_t rows, size_t cols) {
+  return dim3((unsigned)cdiv(cols, 256), (unsigned)std::min<size_t>(rows, 65535));
 }
 
 template <typename T, bool CPLX>
@@ -444,12 +439,12 @@ static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, con
     E* ap = reinterpret_cast<E*>(ws);
     E* bp = reinterpret_cast<E*>((uint8_t*)ap + cdiv(Mp * Kp * sizeof(E), 256) * 256);
     E* cp = reinterpret_cast<E*>((uint8_t*)bp + cdiv(Kp * Np * sizeof(E), 256) * 256);
-    hipLaunchKernelGGL(pad_copy_kernel<E>, dim3(copy_grid<E>(Mp * Kp)), dim3(256), 0, st, ap,
+    hipLaunchKernelGGL(pad_copy_kernel<E>, copy_grid(Mp, Kp), dim3(256), 0, st, ap,
                        reinterpret_cast<const E*>(a), m, k, Mp, Kp);
-    hipLaunchKernelGGL(pad_copy_kernel<E>, dim3(copy_grid<E>(Kp * Np)), dim3(256), 0, st, bp,
+    hipLaunchKernelGGL(pad_copy_kernel<E>, copy_grid(Kp, Np), dim3(256), 0, st, bp,
                        reinterpret_cast<const E*>(b), k, n, Kp, Np);
     if (!p.beta_zero)
-      hipLaunchKernelGGL(pad_copy_kernel<E>, dim3(copy_grid<E>(Mp * Np)), dim3(256), 0, st, cp,
+      hipLaunchKernelGGL(pad_copy_kernel<E>, copy_grid(Mp, Np), dim3(256), 0, st, cp,
                          reinterpret_cast<const E*>(c), m, n, Mp, Np);
     FG_CHECK(hipGetLastError());
     RealArgs<T> q = p;
@@ -462,7 +457,7 @@ static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, con
     const size_t ptiles = (Mp / RB_M) * (Np * mult / RB_N);
     hipLaunchKernelGGL((gemm_real_kernel<T, CPLX, false>), dim3((unsigned)ptiles),
                        dim3(R_THREADS), 0, st, q);
-    hipLaunchKernelGGL(unpad_copy_kernel<E>, dim3(copy_grid<E>(m * n)), dim3(256), 0, st,
+    hipLaunchKernelGGL(unpad_copy_kernel<E>, copy_grid(m, n), dim3(256), 0, st,
                        reinterpret_cast<E*>(c), cp, m, n, Np);
     FG_CHECK(hipGetLastError());
     return FERRAY_OK;
