@@ -281,11 +281,13 @@ __global__ __launch_bounds__(S::THREADS) void mfma_nt_kernel(NtOperands p, Epi e
 // no LDS write instructions.  One such instruction writes 1 KiB of LDS at
 // consecutive 16-byte slots (lane l at slot l), so the wave fills 8 rows of
 // 128 bytes and each lane chooses which 16-byte chunk of its row it fetches:
-// row r keeps chunk c at slot c ^ swz(r), swz(r) = (r & 7) ^ ((r >> 3) & 1),
-// so 16 consecutive rows of one chunk land on distinct banks.  rocprofv3
-// counts no LDS bank conflicts for the 32 x 32 reads (fp8); the 16 x 16
-// reads show 4 conflict cycles per read with this and with the other
-// swizzles tried, with no measurable effect on the time.
+// row r keeps chunk c at slot c ^ swz(r),
+//   swz(r) = ((r >> 1) & 7) ^ ((r & 1) << 2)   (r taken mod 16).
+// Chosen by timing ds_read_b128 with both fragment read patterns over
+// candidate swizzles on the MI355X: this one reads at the full LDS rate for
+// the 16 x 16 pattern (lanes 0-15 on rows 0-15 of one chunk, the other lane
+// groups on the next chunks) and the 32 x 32 one, where (r & 7) ^ ((r >> 3) & 1)
+// ran the 16 x 16 reads at half rate.
 //
 // The pipeline is rotated: a stage's last k-step is multiplied after the
 // barrier that opens the next stage, interleaved with that stage's copy
@@ -373,7 +375,8 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
   // panel: lane l writes slot l & 7 of row l >> 3.  Offsets within the tile
   // are 32-bit (the launcher checks 256 rows fit).
   const int lrow = lane >> 3, lslot = lane & 7;
-  const int gchunk = lslot ^ lrow ^ (wave & 1);  // swz of row 8 (4q + w) + lrow
+  // swz of row 8 (4q + w) + lrow, whose low four bits are 8 (w & 1) + lrow
+  const int gchunk = lslot ^ ((4 * (wave & 1) + (lrow >> 1)) ^ ((lrow & 1) << 2));
   const unsigned a_off = (unsigned)((wave * 8 + lrow) * p.lda + gchunk * 16);
   const unsigned b_off = (unsigned)((wave * 8 + lrow) * p.ldb + gchunk * 16);
   const unsigned a_q = (unsigned)(32 * p.lda), b_q = (unsigned)(32 * p.ldb);
@@ -395,7 +398,7 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
   // Fragment of k-step kk: lane l reads row l % TS of the tile, 16-byte
   // chunk(s) kk * CH + (TS == 32 ? h * 2 + l / 32 : l / 16) (h = second read
   // of an fp8 fragment), at its swizzled slot.
-  const int swz = (lr & 7) ^ ((lr >> 3) & 1);
+  const int swz = ((lr >> 1) & 7) ^ ((lr & 1) << 2);
   int xo[NKK][RD];
 #pragma unroll
   for (int kk = 0; kk < NKK; ++kk)
