@@ -29,6 +29,59 @@ __global__ void trsm_scale_kernel(int m, int n, T alpha, T* __restrict__ b, size
   }
 }
 
+// Lower leaf with LPC threads per column: thread q of column j owns rows
+// q, q + LPC, ..., and the solve runs right-looking, step p finishing x_p on
+// its owner, broadcasting it to the others and subtracting L_ip x_p from
+// every row i > p.  Each row still receives its updates in increasing p,
+// each as a rounded product then a rounded difference, then the scaling by
+// 1 / L_ii: the same operations in the same order as the one-thread-per-
+// column substitution below, so the same bits, with LPC times the threads
+// on the chip.  Rows past m are zero (their L rows and inverse diagonal too)
+// and are never written back.
+template <typename T, int LPC>
+__global__ __launch_bounds__(256) void trsm_leaf_lower_kernel(int m, int n,
+                                                               const T* __restrict__ a,
+                                                               size_t lda, T* __restrict__ b,
+                                                               size_t ldb, int unit_diag) {
+  constexpr int NB = TRSM_NB, R = NB / LPC;
+  __shared__ T As[NB][NB + 1];
+  __shared__ T inv_diag[NB];
+  for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+    const int r = idx / NB, c = idx % NB;
+    As[r][c] = (r < m && c < m) ? a[(size_t)r * lda + c] : T(0);
+  }
+  __syncthreads();
+  if (threadIdx.x < NB) inv_diag[threadIdx.x] = threadIdx.x < m ? T(1) / As[threadIdx.x][threadIdx.x] : T(0);
+  __syncthreads();
+  const int j = (int)((blockIdx.x * blockDim.x + threadIdx.x) / LPC);
+  const int q = threadIdx.x % LPC, lane = threadIdx.x & 63;
+  const bool live = j < n;
+  T v[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int i = q + LPC * t;
+    v[t] = (live && i < m) ? b[(size_t)i * ldb + j] : T(0);
+  }
+#pragma unroll
+  for (int p = 0; p < NB; ++p) {
+    const int owner = p % LPC, tp = p / LPC;
+    T xp = v[tp];
+    if (!unit_diag) xp *= inv_diag[p];
+    if (q == owner) v[tp] = xp;
+    xp = __shfl(xp, (lane & ~(LPC - 1)) | owner, 64);
+#pragma unroll
+    for (int t = tp; t < R; ++t) {
+      const int i = q + LPC * t;
+      if (i > p) v[t] -= As[i][p] * xp;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int i = q + LPC * t;
+    if (live && i < m) b[(size_t)i * ldb + j] = v[t];
+  }
+}
+
 // Leaf solve of an m x m (m <= NB) triangle; thread j owns column j of B.
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void trsm_leaf_kernel(int m, int n, const T* __restrict__ a,
@@ -144,9 +197,12 @@ static int trsm_rec(const TrsmCtx<T>& c, size_t m, const T* a, T* b) {
     if (c.upper)
       hipLaunchKernelGGL((trsm_leaf_kernel<T, false>), dim3(blocks), dim3(256), 0, c.st, (int)m,
                          c.n, a, c.lda, b, c.ldb, (int)c.unit);
+    else if ((size_t)c.n * 4 < (size_t)256 * 256)  // fewer than 256 blocks at 4 per column
+      hipLaunchKernelGGL((trsm_leaf_lower_kernel<T, 8>), dim3((unsigned)cdiv((size_t)c.n * 8, 256)),
+                         dim3(256), 0, c.st, (int)m, c.n, a, c.lda, b, c.ldb, (int)c.unit);
     else
-      hipLaunchKernelGGL((trsm_leaf_kernel<T, true>), dim3(blocks), dim3(256), 0, c.st, (int)m,
-                         c.n, a, c.lda, b, c.ldb, (int)c.unit);
+      hipLaunchKernelGGL((trsm_leaf_lower_kernel<T, 4>), dim3((unsigned)cdiv((size_t)c.n * 4, 256)),
+                         dim3(256), 0, c.st, (int)m, c.n, a, c.lda, b, c.ldb, (int)c.unit);
     FG_CHECK(hipGetLastError());
     return FERRAY_OK;
   }

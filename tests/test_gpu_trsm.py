@@ -2,7 +2,8 @@
 
 Bars:
   * solves that fit one leaf (lower m <= 64, upper m <= 32) are bit-identical
-    to the reference's scalar substitution (same operation order);
+    to the reference's scalar substitution (same operation order), including
+    the lower leaf's split of each column over 4 or 8 threads;
   * larger solves (the trailing updates run on MFMA GEMMs) agree with the
     oracle to 1e-10 relative (f64) / 2e-4 (f32) on well-conditioned systems,
     and their normwise backward error ||A X - alpha B|| / (||A|| ||X||) is
@@ -40,8 +41,9 @@ def _gpu(lower, m, n, alpha, a, lda, b, ldb, unit):
 
 @pytest.mark.parametrize("lower", [True, False])
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
-@pytest.mark.parametrize("m,n", [(1, 1), (5, 3), (32, 9), (64, 300)])
+@pytest.mark.parametrize("m,n", [(1, 1), (5, 3), (32, 9), (64, 300), (47, 16400), (64, 16500)])
 def test_single_leaf_bit_exact(gpu, lower, dt, m, n):
+    # n >= 16384 runs the lower leaf with 4 threads per column, smaller n with 8
     if not lower and m > 32:
         pytest.skip("upper solves above 32 rows take the reference's split order")
     a, b = _system(lower, m, n, dt, 1)
