@@ -63,7 +63,6 @@ struct EpiI32 {
 // order-independent, so the result is exact whatever order the adds land
 // in.  The first chunk's row z = 0 also adds the sum corrections.
 struct EpiI16Atomic {
-  static constexpr bool staged_loads = true;
   int32_t* c;
   int M, N;
   const int32_t* colsum;   // 128*colsum(B) term, or null
@@ -79,6 +78,28 @@ struct EpiI16Atomic {
       v += kconst;
     }
     atomicAdd(reinterpret_cast<unsigned int*>(c + (size_t)row * N + col), v);
+  }
+};
+
+// The same i16 product when the four plane products were combined in the
+// kernel (256 x 256 tiles): one store per element, C read when accumulating
+// over K chunks or into the caller's C.
+struct EpiI16 {
+  int32_t* c;
+  int M, N;
+  int read_c;
+  const int32_t* colsum;   // 128*colsum(B) term, or null
+  const int32_t* rowsum;   // 128*rowsum(A) term, or null
+  uint32_t kconst;         // -16384*k, first chunk only
+  __device__ __forceinline__ void operator()(int row, int col, int, int acc) const {
+    if (row >= M || col >= N) return;
+    uint32_t v = (uint32_t)acc;
+    if (colsum) v += 128u * (uint32_t)colsum[col];
+    if (rowsum) v += 128u * (uint32_t)rowsum[row];
+    v += kconst;
+    int32_t* dst = c + (size_t)row * N + col;
+    if (read_c) v += (uint32_t)*dst;
+    *dst = (int32_t)v;
   }
 };
 
@@ -428,21 +449,30 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
         rc = FERRAY_ERR_INVALID;  // unreachable: run_nt routes k > 2^16 to run_quant_long
       }
     } else if (kind == NT_I16) {
-      // All four byte-plane products in one launch (hi*hi<<16, hi*lo'<<8,
-      // lo'*hi<<8, lo'*lo'), adding into C; C starts at zero unless the
-      // call accumulates.
-      if (first && !accumulate) {
-        hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(m * n)), dim3(256), 0, st,
-                           reinterpret_cast<uint32_t*>(c), m * n);
-        FG_CHECK(hipGetLastError());
-      }
       NtOperands r = q;
       r.a_plane = (size_t)((ws + o_a1) - (ws + o_a0));
       r.b_plane = (size_t)((ws + o_bt1) - (ws + o_bt0));
-      r.cross = 1;
-      EpiI16Atomic e{reinterpret_cast<int32_t*>(c), M, N, first ? csp : nullptr,
-                     first ? rsp : nullptr, first ? (uint32_t)(0u - 16384u * (uint32_t)k) : 0u};
-      rc = launch_mfma_nt<ELEM_I8, S, false>(r, 4, e, st);
+      const int32_t* cs = first ? csp : nullptr;
+      const int32_t* rs = first ? rsp : nullptr;
+      const uint32_t kc = first ? (uint32_t)(0u - 16384u * (uint32_t)k) : 0u;
+      if constexpr (S::BM == 256) {
+        // The four byte-plane products combined in the kernel (Horner on
+        // the accumulators), one store per element.
+        EpiI16 e{reinterpret_cast<int32_t*>(c), M, N, read_c, cs, rs, kc};
+        rc = launch_mfma_nt_i16_planes(r, e, st);
+      } else {
+        // All four byte-plane products in one launch (hi*hi<<16, hi*lo'<<8,
+        // lo'*hi<<8, lo'*lo'), adding into C; C starts at zero unless the
+        // call accumulates.
+        if (first && !accumulate) {
+          hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(m * n)), dim3(256), 0, st,
+                             reinterpret_cast<uint32_t*>(c), m * n);
+          FG_CHECK(hipGetLastError());
+        }
+        r.cross = 1;
+        EpiI16Atomic e{reinterpret_cast<int32_t*>(c), M, N, cs, rs, kc};
+        rc = launch_mfma_nt<ELEM_I8, S, false>(r, 4, e, st);
+      }
     } else {
       EpiI32 e{reinterpret_cast<int32_t*>(c), M, N, read_c,
                (kind == NT_U8 && first) ? csp : nullptr};

@@ -336,18 +336,15 @@ struct NtDmaAcc<ELEM_F16> {
   }
 };
 
-// An epilogue that declares `static constexpr bool staged_loads = true`
-// keeps the register-staged kernel for the 256 x 256 shape (the i16 product
-// with its four byte planes and atomic epilogue measured 5 % faster there).
-template <class E, class = void>
-struct NtStaged : std::false_type {};
-template <class E>
-struct NtStaged<E, std::void_t<decltype(E::staged_loads)>> : std::bool_constant<E::staged_loads> {};
-
 typedef __attribute__((address_space(1))) void nt_gvoid;
 typedef __attribute__((address_space(3))) void nt_lvoid;
 
-template <int ELEM, class Epi>
+// SEQ = 1 runs the i16 product of byte planes (A planes 0/1 = high/low, B
+// the same) in one block: hi*hi, then the accumulators shifted left by 8,
+// hi*lo and lo*hi, shifted again, lo*lo — Horner's rule for
+// hi*hi*2^16 + (hi*lo + lo*hi)*2^8 + lo*lo, exact mod 2^32 in the wrapping
+// i32 accumulators — so the epilogue stores once (no atomics).
+template <int ELEM, class Epi, int SEQ = 0>
 __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi) {
   using Acc = NtDmaAcc<ELEM>;
   using acc_t = typename Acc::type;
@@ -489,22 +486,45 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
   using F = std::integral_constant<bool, false>;
 
   const int nk = p.kbytes / KB;
-  issue(0, 0);
-  __syncthreads();
-  if (nk > 1)
-    body(F{}, T{}, 0, KB);
-  else
-    body(F{}, F{}, 0, 0);
-  int kt = 1;
-  for (; kt + 1 < nk; ++kt) {
-    barrier();
-    body(T{}, T{}, kt & 1, (kt + 1) * KB);
+  auto kloop = [&]() __attribute__((always_inline)) {
+    issue(0, 0);
+    __syncthreads();
+    if (nk > 1)
+      body(F{}, T{}, 0, KB);
+    else
+      body(F{}, F{}, 0, 0);
+    int kt = 1;
+    for (; kt + 1 < nk; ++kt) {
+      barrier();
+      body(T{}, T{}, kt & 1, (kt + 1) * KB);
+    }
+    if (kt < nk) {
+      barrier();
+      body(T{}, F{}, kt & 1, 0);
+    }
+    mma(pa, pb);
+  };
+  if constexpr (SEQ == 0) {
+    kloop();
+  } else {
+    static_assert(ELEM == ELEM_I8, "plane sequence is the i16 product");
+    const uint8_t* A0 = A;
+    const uint8_t* B0 = B;
+    auto shift8 = [&]() __attribute__((always_inline)) {
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) acc[i][j] <<= 8;
+    };
+    // hi*hi, shift, hi*lo, lo*hi, shift, lo*lo
+    for (int ps = 0; ps < 4; ++ps) {
+      A = A0 + (ps >> 1) * p.a_plane;
+      B = B0 + ((ps == 1 || ps == 3) ? p.b_plane : 0);
+      if (ps) barrier();  // every wave is done with the buffers before the next copies
+      kloop();
+      if (ps == 0 || ps == 2) shift8();
+    }
   }
-  if (kt < nk) {
-    barrier();
-    body(T{}, F{}, kt & 1, 0);
-  }
-  mma(pa, pb);
 
   // Result element r of lane l: 32 x 32 tiles as in the NT kernel; 16 x 16
   // tiles row 4 (l / 16) + r, column l % 16.
@@ -523,11 +543,32 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
       }
 }
 
+// The i16 byte-plane product on 256 x 256 tiles: A planes 0 / 1 at p.a and
+// p.a + p.a_plane, B planes likewise; one launch, one store per element.
+template <class Epi>
+inline int launch_mfma_nt_i16_planes(const NtOperands& p, const Epi& epi, hipStream_t st) {
+  const size_t tiles = (size_t)p.tiles_m * p.tiles_n;
+  if (tiles == 0) return FERRAY_OK;
+  if (tiles > 0x7fffffff || p.kbytes <= 0 || p.kbytes % 128) return FERRAY_ERR_INVALID;
+  if (256 * (p.lda > p.ldb ? p.lda : p.ldb) >= ((size_t)1 << 32)) return FERRAY_ERR_INVALID;
+  auto kern = mfma_nt_dma_kernel<ELEM_I8, Epi, 1>;
+  constexpr int lds = 2 * 512 * 128;
+  static bool attr_set = false;  // every caller sets the same value; a race is benign
+  if (!attr_set) {
+    FG_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 lds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)tiles, 1), dim3(256), lds, st, p, epi);
+  FG_CHECK(hipGetLastError());
+  return FERRAY_OK;
+}
+
 // Launches the NT main loop over `planes` independent plane pairs.  The
 // 256 x 256 shape runs the direct-to-LDS kernel when 256 rows of either
-// operand span less than 4 GiB (its in-tile offsets are 32-bit), A needs no
-// sign flip (the copies move bytes unmodified; flipping fragments after the
-// reads spills registers) and the epilogue does not ask for staged loads.
+// operand span less than 4 GiB (its in-tile offsets are 32-bit) and A needs
+// no sign flip (the copies move bytes unmodified; flipping fragments after
+// the reads spills registers).
 template <int ELEM, class S, bool XOR_A, class Epi>
 inline int launch_mfma_nt(const NtOperands& p, int planes, const Epi& epi, hipStream_t st) {
   const size_t tiles = (size_t)p.tiles_m * p.tiles_n;
@@ -535,7 +576,7 @@ inline int launch_mfma_nt(const NtOperands& p, int planes, const Epi& epi, hipSt
   if (tiles > 0x7fffffff || planes > 65535) return FERRAY_ERR_INVALID;
   if (p.kbytes <= 0 || p.kbytes % S::KB) return FERRAY_ERR_INVALID;
   const dim3 grid((unsigned)tiles, (unsigned)planes);
-  if constexpr (S::BM == 256 && S::BN == 256 && S::KB == 128 && !XOR_A && !NtStaged<Epi>::value) {
+  if constexpr (S::BM == 256 && S::BN == 256 && S::KB == 128 && !XOR_A) {
     if (256 * (p.lda > p.ldb ? p.lda : p.ldb) < ((size_t)1 << 32)) {
       auto kern = mfma_nt_dma_kernel<ELEM, Epi>;
       constexpr int lds = 2 * 512 * 128;
