@@ -33,7 +33,7 @@
 
 namespace fg {
 
-constexpr int RB_M = 128, RB_N = 128, R_THREADS = RT_THREADS, R_GROUP_M = 8;
+constexpr int RB_M = 128, RB_N = 128, R_THREADS = RT_THREADS, R_GROUP_M = 4;
 
 // Whether a product of `tiles` 128 x 128 tiles runs faster on 64 x 64 tiles:
 // below 256 tiles the 128 x 128 grid leaves CUs idle, and up to 2048 tiles a
