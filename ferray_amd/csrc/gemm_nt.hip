@@ -42,6 +42,20 @@ constexpr size_t NT_K_CHUNK = 1u << 16;
 
 // i32 output of the i8 products: v = acc + 128*colsum[col] (if colsum; u8 A
 // was shifted by -128) + (old C if read_c), all wrapping.
+template <bool READ_C, bool COLSUM>
+struct EpiI32Fast {  // EpiI32 inside C, flags fixed
+  int32_t* c;
+  int N;
+  const int32_t* colsum;
+  __device__ __forceinline__ void operator()(int row, int col, int, int acc) const {
+    uint32_t v = (uint32_t)acc;
+    if (COLSUM) v += 128u * (uint32_t)colsum[col];
+    int32_t* dst = c + (size_t)row * N + col;
+    if (READ_C) v += (uint32_t)*dst;
+    *dst = (int32_t)v;
+  }
+};
+
 struct EpiI32 {
   int32_t* c;
   int M, N;
@@ -54,6 +68,19 @@ struct EpiI32 {
     int32_t* dst = c + (size_t)row * N + col;
     if (read_c) v += (uint32_t)*dst;
     *dst = (int32_t)v;
+  }
+  __device__ __forceinline__ bool interior(int r0, int c0, int rows, int cols) const {
+    return r0 + rows <= M && c0 + cols <= N;
+  }
+  template <class F>
+  __device__ __forceinline__ void with_fast(F&& f) const {
+    if (read_c) {
+      if (colsum) f(EpiI32Fast<true, true>{c, N, colsum});
+      else f(EpiI32Fast<true, false>{c, N, colsum});
+    } else {
+      if (colsum) f(EpiI32Fast<false, true>{c, N, colsum});
+      else f(EpiI32Fast<false, false>{c, N, colsum});
+    }
   }
 };
 
@@ -106,6 +133,24 @@ struct EpiI16 {
 // quantized_matmul (mod.rs:524-573): raw = acc + 128*csb (A was shifted by
 // -128); corrected = raw - azp*csb - bzp*rsa + k*azp*bzp (wrapping);
 // out = (scale_a*scale_b) * (float)corrected, written or added.
+template <bool ACC>
+struct EpiQuantFast {  // EpiQuant inside C, accumulate fixed
+  float* c;
+  int N;
+  const int32_t* colsum;
+  const int32_t* rowsum;
+  uint32_t azp, bzp, kab;
+  float scale;
+  __device__ __forceinline__ void operator()(int row, int col, int, int acc) const {
+    const uint32_t cs = (uint32_t)colsum[col];
+    const uint32_t v = (uint32_t)acc + 128u * cs - azp * cs - bzp * (uint32_t)rowsum[row] + kab;
+    float out = scale * (float)(int32_t)v;
+    float* dst = c + (size_t)row * N + col;
+    if (ACC) out = *dst + out;
+    *dst = out;
+  }
+};
+
 struct EpiQuant {
   float* c;
   int M, N;
@@ -115,17 +160,34 @@ struct EpiQuant {
   uint32_t azp, bzp, kab;
   float scale;
   __device__ __forceinline__ void operator()(int row, int col, int, int acc) const {
-    // The sums are read unconditionally (clamped indices, always in bounds)
-    // so the compiler can issue every element's loads ahead of the stores.
-    const uint32_t cs = (uint32_t)colsum[min(col, N - 1)];
-    const uint32_t rs = (uint32_t)rowsum[min(row, M - 1)];
     if (row >= M || col >= N) return;
+    const uint32_t cs = (uint32_t)colsum[col];
     const uint32_t raw = (uint32_t)acc + 128u * cs;
-    const uint32_t v = raw - azp * cs - bzp * rs + kab;
+    const uint32_t v = raw - azp * cs - bzp * (uint32_t)rowsum[row] + kab;
     float out = scale * (float)(int32_t)v;
     float* dst = c + (size_t)row * N + col;
     if (accumulate) out = *dst + out;
     *dst = out;
+  }
+  __device__ __forceinline__ bool interior(int r0, int c0, int rows, int cols) const {
+    return r0 + rows <= M && c0 + cols <= N;
+  }
+  template <class F>
+  __device__ __forceinline__ void with_fast(F&& f) const {
+    if (accumulate)
+      f(EpiQuantFast<true>{c, N, colsum, rowsum, azp, bzp, kab, scale});
+    else
+      f(EpiQuantFast<false>{c, N, colsum, rowsum, azp, bzp, kab, scale});
+  }
+};
+
+template <bool ACC>
+struct EpiF32Fast {  // EpiF32 inside C, accumulate fixed
+  float* c;
+  int N;
+  __device__ __forceinline__ void operator()(int row, int col, int, float acc) const {
+    float* dst = c + (size_t)row * N + col;
+    *dst = ACC ? *dst + acc : acc;
   }
 };
 
@@ -137,6 +199,16 @@ struct EpiF32 {
     if (row >= M || col >= N) return;
     float* dst = c + (size_t)row * N + col;
     *dst = accumulate ? *dst + acc : acc;
+  }
+  __device__ __forceinline__ bool interior(int r0, int c0, int rows, int cols) const {
+    return r0 + rows <= M && c0 + cols <= N;
+  }
+  template <class F>
+  __device__ __forceinline__ void with_fast(F&& f) const {
+    if (accumulate)
+      f(EpiF32Fast<true>{c, N});
+    else
+      f(EpiF32Fast<false>{c, N});
   }
 };
 

@@ -35,6 +35,7 @@
 // (row, col, plane, value); it owns all stores and bounds checks.
 #pragma once
 #include <type_traits>
+#include <utility>
 
 #include "common.hpp"
 
@@ -142,6 +143,20 @@ __device__ __forceinline__ void nt_tile(int tiles_m, int tiles_n, int& tm, int& 
   const int gsz = min(tiles_m - first_m, G), in_group = bid - group * per_group;
   tm = first_m + in_group % gsz;
   tn = in_group / gsz;
+}
+
+// A copy of an accumulator made by a VALU move into an ordinary VGPR.  The
+// NT kernels' epilogues take every result through one.  Left to itself the
+// compiler stores results straight from the accumulator registers, and that
+// form has written one wrong word per block tile and wave (the same constant
+// in every tile, one result register) on some MI355X boards, where the
+// copied form has not.
+template <class T>
+__device__ __forceinline__ T nt_vgpr(T x) {
+  static_assert(sizeof(T) == 4, "one 32-bit register");
+  T y;
+  asm("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+  return y;
 }
 
 // Result element r of lane l in a 32x32 tile: row 8*(r/4) + 4*(l/32) + r%4,
@@ -272,7 +287,8 @@ __global__ __launch_bounds__(S::THREADS) void mfma_nt_kernel(NtOperands p, Epi e
     for (int j = 0; j < TJ; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r)
-        epi(row0 + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3), col0 + j * 32 + lr, z, acc[i][j][r]);
+        epi(row0 + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3), col0 + j * 32 + lr, z,
+            nt_vgpr(acc[i][j][r]));
 }
 
 // ---------------------------------------------------------------------------
@@ -338,6 +354,16 @@ struct NtDmaAcc<ELEM_F16> {
                                                   __builtin_bit_cast(h8, b), c, 0, 0, 0);
   }
 };
+
+// An epilogue may offer an unchecked form for block tiles inside C:
+// `bool interior(row0, col0, rows, cols)` and `with_fast(f)`, which calls f
+// with a functor that stores without bounds checks, its run-time flags
+// turned into template parameters.
+template <class E, class = void>
+struct NtFastEpi : std::false_type {};
+template <class E>
+struct NtFastEpi<E, std::void_t<decltype(std::declval<const E&>().interior(0, 0, 0, 0))>>
+    : std::true_type {};
 
 typedef __attribute__((address_space(1))) void nt_gvoid;
 typedef __attribute__((address_space(3))) void nt_lvoid;
@@ -532,18 +558,31 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
   // Result element r of lane l: 32 x 32 tiles as in the NT kernel; 16 x 16
   // tiles row 4 (l / 16) + r, column l % 16.
   const int row0 = tm * BM + wm0, col0 = tn * BN + wn0;
+  auto store_all = [&](const auto& e) __attribute__((always_inline)) {
 #pragma unroll
-  for (int i = 0; i < TI; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < TJ; ++j)
+      for (int j = 0; j < TJ; ++j)
 #pragma unroll
-      for (int r = 0; r < ROUT; ++r) {
-        if constexpr (TS == 32)
-          epi(row0 + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3), col0 + j * 32 + lr, z,
-              acc[i][j][r]);
-        else
-          epi(row0 + i * 16 + 4 * lk + r, col0 + j * 16 + lr, z, acc[i][j][r]);
-      }
+        for (int r = 0; r < ROUT; ++r) {
+          const auto v = nt_vgpr(acc[i][j][r]);
+          if constexpr (TS == 32)
+            e(row0 + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3), col0 + j * 32 + lr, z, v);
+          else
+            e(row0 + i * 16 + 4 * lk + r, col0 + j * 16 + lr, z, v);
+        }
+  };
+  // Block tiles wholly inside C take the epilogue's unchecked form when it
+  // has one: no per-element bounds or flag branches, so the compiler can
+  // issue the epilogue's loads together.
+  if constexpr (NtFastEpi<Epi>::value) {
+    if (epi.interior(tm * BM, tn * BN, BM, BN))
+      epi.with_fast(store_all);
+    else
+      store_all(epi);
+  } else {
+    store_all(epi);
+  }
 }
 
 // The i16 byte-plane product on 256 x 256 tiles: A planes 0 / 1 at p.a and
