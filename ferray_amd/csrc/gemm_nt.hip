@@ -108,28 +108,6 @@ struct EpiI16Atomic {
   }
 };
 
-// The same i16 product when the four plane products were combined in the
-// kernel (256 x 256 tiles): one store per element, C read when accumulating
-// over K chunks or into the caller's C.
-struct EpiI16 {
-  int32_t* c;
-  int M, N;
-  int read_c;
-  const int32_t* colsum;   // 128*colsum(B) term, or null
-  const int32_t* rowsum;   // 128*rowsum(A) term, or null
-  uint32_t kconst;         // -16384*k, first chunk only
-  __device__ __forceinline__ void operator()(int row, int col, int, int acc) const {
-    if (row >= M || col >= N) return;
-    uint32_t v = (uint32_t)acc;
-    if (colsum) v += 128u * (uint32_t)colsum[col];
-    if (rowsum) v += 128u * (uint32_t)rowsum[row];
-    v += kconst;
-    int32_t* dst = c + (size_t)row * N + col;
-    if (read_c) v += (uint32_t)*dst;
-    *dst = (int32_t)v;
-  }
-};
-
 // quantized_matmul (mod.rs:524-573): raw = acc + 128*csb (A was shifted by
 // -128); corrected = raw - azp*csb - bzp*rsa + k*azp*bzp (wrapping);
 // out = (scale_a*scale_b) * (float)corrected, written or added.
@@ -527,24 +505,17 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
       const int32_t* cs = first ? csp : nullptr;
       const int32_t* rs = first ? rsp : nullptr;
       const uint32_t kc = first ? (uint32_t)(0u - 16384u * (uint32_t)k) : 0u;
-      if constexpr (S::BM == 256) {
-        // The four byte-plane products combined in the kernel (Horner on
-        // the accumulators), one store per element.
-        EpiI16 e{reinterpret_cast<int32_t*>(c), M, N, read_c, cs, rs, kc};
-        rc = launch_mfma_nt_i16_planes(r, e, st);
-      } else {
-        // All four byte-plane products in one launch (hi*hi<<16, hi*lo'<<8,
-        // lo'*hi<<8, lo'*lo'), adding into C; C starts at zero unless the
-        // call accumulates.
-        if (first && !accumulate) {
-          hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(m * n)), dim3(256), 0, st,
-                             reinterpret_cast<uint32_t*>(c), m * n);
-          FG_CHECK(hipGetLastError());
-        }
-        r.cross = 1;
-        EpiI16Atomic e{reinterpret_cast<int32_t*>(c), M, N, cs, rs, kc};
-        rc = launch_mfma_nt<ELEM_I8, S, false>(r, 4, e, st);
+      // All four byte-plane products in one launch (hi*hi<<16, hi*lo'<<8,
+      // lo'*hi<<8, lo'*lo'), adding into C; C starts at zero unless the
+      // call accumulates.
+      if (first && !accumulate) {
+        hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(m * n)), dim3(256), 0, st,
+                           reinterpret_cast<uint32_t*>(c), m * n);
+        FG_CHECK(hipGetLastError());
       }
+      r.cross = 1;
+      EpiI16Atomic e{reinterpret_cast<int32_t*>(c), M, N, cs, rs, kc};
+      rc = launch_mfma_nt<ELEM_I8, S, false>(r, 4, e, st);
     } else {
       EpiI32 e{reinterpret_cast<int32_t*>(c), M, N, read_c,
                (kind == NT_U8 && first) ? csp : nullptr};

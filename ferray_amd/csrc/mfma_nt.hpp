@@ -313,10 +313,11 @@ __global__ __launch_bounds__(S::THREADS) void mfma_nt_kernel(NtOperands p, Epi e
 // issue, while its first fragment reads are in flight.  The matrix pipe then
 // never waits for the barrier or the first LDS reads of a stage.
 //
-// i8 / bf16 / f16 use the 16 x 16 MFMAs (16x16x64 i8, 16x16x32 bf16/f16):
-// measured 2-3 % faster here than the 32 x 32 forms.  fp8 / bf8 use the
-// 32x32x64 f8f6f4 form of the NT kernel above, so their results do not
-// depend on the tile shape.  Integer results are exact either way.
+// i8 uses the 16x16x64 MFMA: measured 2-3 % faster here than 32x32x32, and
+// integer results are exact either way.  bf16 / f16 / fp8 / bf8 use the NT
+// kernel's 32 x 32 forms.  The 16x16x32 bf16 / f16 form ran 7 % faster but
+// spilled accumulators to scratch; every kernel here keeps its accumulators
+// in registers and stores results from VGPRs (nt_vgpr).
 template <int ELEM>
 struct NtDmaAcc {  // 32 x 32 tiles: the NT kernel's instruction
   using A = NtAcc<ELEM>;
@@ -334,26 +335,6 @@ struct NtDmaAcc<ELEM_I8> {
     return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
   }
 };
-template <>
-struct NtDmaAcc<ELEM_BF16> {
-  using type = f4;
-  using frag = i4;
-  static constexpr int TS = 16, KSTEP = 64, READS = 1;
-  static __device__ __forceinline__ type step(i4 a, i4 b, type c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a),
-                                                   __builtin_bit_cast(bf8, b), c, 0, 0, 0);
-  }
-};
-template <>
-struct NtDmaAcc<ELEM_F16> {
-  using type = f4;
-  using frag = i4;
-  static constexpr int TS = 16, KSTEP = 64, READS = 1;
-  static __device__ __forceinline__ type step(i4 a, i4 b, type c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a),
-                                                  __builtin_bit_cast(h8, b), c, 0, 0, 0);
-  }
-};
 
 // An epilogue may offer an unchecked form for block tiles inside C:
 // `bool interior(row0, col0, rows, cols)` and `with_fast(f)`, which calls f
@@ -368,12 +349,7 @@ struct NtFastEpi<E, std::void_t<decltype(std::declval<const E&>().interior(0, 0,
 typedef __attribute__((address_space(1))) void nt_gvoid;
 typedef __attribute__((address_space(3))) void nt_lvoid;
 
-// SEQ = 1 runs the i16 product of byte planes (A planes 0/1 = high/low, B
-// the same) in one block: hi*hi, then the accumulators shifted left by 8,
-// hi*lo and lo*hi, shifted again, lo*lo — Horner's rule for
-// hi*hi*2^16 + (hi*lo + lo*hi)*2^8 + lo*lo, exact mod 2^32 in the wrapping
-// i32 accumulators — so the epilogue stores once (no atomics).
-template <int ELEM, class Epi, int SEQ = 0>
+template <int ELEM, class Epi>
 __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi) {
   using Acc = NtDmaAcc<ELEM>;
   using acc_t = typename Acc::type;
@@ -533,27 +509,7 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
     }
     mma(pa, pb);
   };
-  if constexpr (SEQ == 0) {
-    kloop();
-  } else {
-    static_assert(ELEM == ELEM_I8, "plane sequence is the i16 product");
-    const uint8_t* A0 = A;
-    const uint8_t* B0 = B;
-    auto shift8 = [&]() __attribute__((always_inline)) {
-#pragma unroll
-      for (int i = 0; i < TI; ++i)
-#pragma unroll
-        for (int j = 0; j < TJ; ++j) acc[i][j] <<= 8;
-    };
-    // hi*hi, shift, hi*lo, lo*hi, shift, lo*lo
-    for (int ps = 0; ps < 4; ++ps) {
-      A = A0 + (ps >> 1) * p.a_plane;
-      B = B0 + ((ps == 1 || ps == 3) ? p.b_plane : 0);
-      if (ps) barrier();  // every wave is done with the buffers before the next copies
-      kloop();
-      if (ps == 0 || ps == 2) shift8();
-    }
-  }
+  kloop();
 
   // Result element r of lane l: 32 x 32 tiles as in the NT kernel; 16 x 16
   // tiles row 4 (l / 16) + r, column l % 16.
@@ -583,27 +539,6 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
   } else {
     store_all(epi);
   }
-}
-
-// The i16 byte-plane product on 256 x 256 tiles: A planes 0 / 1 at p.a and
-// p.a + p.a_plane, B planes likewise; one launch, one store per element.
-template <class Epi>
-inline int launch_mfma_nt_i16_planes(const NtOperands& p, const Epi& epi, hipStream_t st) {
-  const size_t tiles = (size_t)p.tiles_m * p.tiles_n;
-  if (tiles == 0) return FERRAY_OK;
-  if (tiles > 0x7fffffff || p.kbytes <= 0 || p.kbytes % 128) return FERRAY_ERR_INVALID;
-  if (256 * (p.lda > p.ldb ? p.lda : p.ldb) >= ((size_t)1 << 32)) return FERRAY_ERR_INVALID;
-  auto kern = mfma_nt_dma_kernel<ELEM_I8, Epi, 1>;
-  constexpr int lds = 2 * 512 * 128;
-  static bool attr_set = false;  // every caller sets the same value; a race is benign
-  if (!attr_set) {
-    FG_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 lds));
-    attr_set = true;
-  }
-  hipLaunchKernelGGL(kern, dim3((unsigned)tiles, 1), dim3(256), lds, st, p, epi);
-  FG_CHECK(hipGetLastError());
-  return FERRAY_OK;
 }
 
 // Launches the NT main loop over `planes` independent plane pairs.  The
