@@ -25,6 +25,14 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(3072, 2816, 640), (3000, 2900, 700)]
 
 
+def _same(c, want, n):
+    """Exact equality; on a mismatch, name the count, the first element (row,
+    column) and its got / wanted values."""
+    bad = np.flatnonzero(c != want)
+    assert bad.size == 0, (f"{bad.size} wrong, first at {divmod(int(bad[0]), n)}: "
+                           f"got {c[bad[0]]}, want {want[bad[0]]}")
+
+
 def _exact(a, b, m, n, k):
     return wrap_i32(np.rint(a.reshape(m, k).astype(np.float64) @
                             b.reshape(k, n).astype(np.float64)).astype(np.int64)).ravel()
@@ -38,11 +46,11 @@ def test_i8_signed_large(gpu, m, n, k):
     ref = _exact(a, b, m, n, k)
     c = np.zeros(m * n, np.int32)
     assert gpu.gemm_i8_signed(m, n, k, a, b, c, False)
-    assert np.array_equal(c, ref)
+    _same(c, ref, n)
     c0 = r.integers(-2 ** 31, 2 ** 31, m * n).astype(np.int32)
     c = c0.copy()
     assert gpu.gemm_i8_signed(m, n, k, a, b, c, True)
-    assert np.array_equal(c, wrap_i32(ref.astype(np.int64) + c0))
+    _same(c, wrap_i32(ref.astype(np.int64) + c0), n)
 
 
 @pytest.mark.parametrize("m,n,k", SHAPES)
@@ -52,7 +60,7 @@ def test_u8_i8_large(gpu, m, n, k):
     b = r.integers(-128, 128, k * n).astype(np.int8)
     c = np.zeros(m * n, np.int32)
     assert gpu.gemm_i8(m, n, k, a, b, c, False)
-    assert np.array_equal(c, _exact(a, b, m, n, k))
+    _same(c, _exact(a, b, m, n, k), n)
 
 
 @pytest.mark.parametrize("m,n,k", SHAPES)
@@ -63,11 +71,11 @@ def test_i16_large(gpu, m, n, k):
     ref = _exact(a, b, m, n, k)
     c = np.zeros(m * n, np.int32)
     assert gpu.gemm_i16(m, n, k, a, b, c, False)
-    assert np.array_equal(c, ref)
+    _same(c, ref, n)
     c0 = r.integers(-2 ** 31, 2 ** 31, m * n).astype(np.int32)
     c = c0.copy()
     assert gpu.gemm_i16(m, n, k, a, b, c, True)
-    assert np.array_equal(c, wrap_i32(ref.astype(np.int64) + c0))
+    _same(c, wrap_i32(ref.astype(np.int64) + c0), n)
 
 
 @pytest.mark.parametrize("m,n,k", SHAPES)
@@ -84,7 +92,7 @@ def test_quantized_large(gpu, m, n, k, acc):
     ref = (c0 + out) if acc else out
     c = c0.copy()
     assert gpu.quantized_matmul(m, n, k, a, azp, sa, b, bzp, sb, c, acc)
-    assert np.array_equal(c, ref.astype(np.float32))
+    _same(c, ref.astype(np.float32), n)
 
 
 @pytest.mark.parametrize("m,n,k", SHAPES)
@@ -119,4 +127,4 @@ def test_e4m3_integer_valued_large(gpu, m, n, k):
     ref = (table[a].reshape(m, k) @ table[b].reshape(k, n)).ravel()
     c = np.zeros(m * n, np.float32)
     assert gpu.gemm_e4m3_f32(m, n, k, a, b, c, False)
-    assert np.array_equal(c.astype(np.float64), ref)
+    _same(c.astype(np.float64), ref, n)
