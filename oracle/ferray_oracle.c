@@ -432,65 +432,116 @@ void orc_gemm_fp8_f64(size_t m, size_t n, size_t k, const uint8_t *a, const uint
 /* Accurate dot products with TwoProd (fma) + TwoSum (Knuth), i.e. the
  * compensated "Dot2" of Ogita, Rump & Oishi (2005): the result is as if
  * computed in twice the working precision and then rounded.  Computes
- * C_ref[r, c] for the selected rows and columns (hi and lo parts). */
+ * C_ref[r, c] for the selected rows and columns (hi and lo parts).  Rows are
+ * split over up to min(nproc, 16) threads; each element's arithmetic is the
+ * same sequential loop whatever the split, so the result does not depend on
+ * the thread count. */
+typedef struct {
+  int cplx;
+  size_t n, k, r0, r1, ncols;
+  const double *a, *b;
+  const int64_t *rows, *cols;
+  double *out_hi, *out_lo;
+} orc_dd_task;
+
+static void orc_dd_rows_f64(const orc_dd_task *t) {
+  for (size_t ri = t->r0; ri < t->r1; ++ri) {
+    const double *ar = t->a + (size_t)t->rows[ri] * t->k;
+    for (size_t ci = 0; ci < t->ncols; ++ci) {
+      size_t j = (size_t)t->cols[ci];
+      double s = 0.0, e = 0.0;
+      for (size_t p = 0; p < t->k; ++p) {
+        double x = ar[p], y = t->b[p * t->n + j];
+        double pr = x * y;
+        double pe = fma(x, y, -pr);
+        double u = s + pr;
+        double z = u - s;
+        double se = (s - (u - z)) + (pr - z);
+        s = u;
+        e += se + pe;
+      }
+      double hi = s + e;
+      double lo = e - (hi - s);
+      t->out_hi[ri * t->ncols + ci] = hi;
+      t->out_lo[ri * t->ncols + ci] = lo;
+    }
+  }
+}
+
+/* Complex f64: C = A.B with the real part sum(xr yr) + sum(-xi yi) and the
+ * imaginary part sum(xr yi) + sum(xi yr), each one compensated sum. */
+static void orc_dd_rows_c64(const orc_dd_task *t) {
+  for (size_t ri = t->r0; ri < t->r1; ++ri) {
+    const double *ar = t->a + 2 * (size_t)t->rows[ri] * t->k;
+    for (size_t ci = 0; ci < t->ncols; ++ci) {
+      size_t j = (size_t)t->cols[ci];
+      double s[2] = {0, 0}, e[2] = {0, 0};
+      for (size_t p = 0; p < t->k; ++p) {
+        double xr = ar[2 * p], xi = ar[2 * p + 1];
+        double yr = t->b[2 * (p * t->n + j)], yi = t->b[2 * (p * t->n + j) + 1];
+        double terms[2][2][2] = {{{xr, yr}, {-xi, yi}}, {{xr, yi}, {xi, yr}}};
+        for (int q = 0; q < 2; ++q)
+          for (int w = 0; w < 2; ++w) {
+            double x = terms[q][w][0], y = terms[q][w][1];
+            double pr = x * y, pe = fma(x, y, -pr);
+            double u = s[q] + pr, z = u - s[q];
+            double se = (s[q] - (u - z)) + (pr - z);
+            s[q] = u;
+            e[q] += se + pe;
+          }
+      }
+      t->out_hi[2 * (ri * t->ncols + ci)] = s[0] + e[0];
+      t->out_hi[2 * (ri * t->ncols + ci) + 1] = s[1] + e[1];
+    }
+  }
+}
+
+static void *orc_dd_worker(void *arg) {
+  orc_dd_task *t = (orc_dd_task *)arg;
+  if (t->cplx) orc_dd_rows_c64(t);
+  else orc_dd_rows_f64(t);
+  return NULL;
+}
+
+static void orc_dd_run(orc_dd_task base, size_t nrows) {
+  long np = sysconf(_SC_NPROCESSORS_ONLN);
+  size_t workers = np > 16 ? 16 : (size_t)(np > 0 ? np : 1);
+  if (workers > nrows) workers = nrows ? nrows : 1;
+  orc_dd_task t[16];
+  pthread_t th[16];
+  size_t per = (nrows + workers - 1) / workers;
+  size_t nt = 0;
+  for (size_t r0 = 0; r0 < nrows; r0 += per, ++nt) {
+    t[nt] = base;
+    t[nt].r0 = r0;
+    t[nt].r1 = r0 + per < nrows ? r0 + per : nrows;
+    if (pthread_create(&th[nt], NULL, orc_dd_worker, &t[nt]) != 0) {
+      orc_dd_worker(&t[nt]);
+      th[nt] = 0;
+    }
+  }
+  for (size_t i = 0; i < nt; ++i)
+    if (th[i]) pthread_join(th[i], NULL);
+}
+
 void orc_dd_gemm_f64_sampled(size_t m, size_t n, size_t k, const double *a,
                              const double *b, size_t nrows,
                              const int64_t *rows, size_t ncols,
                              const int64_t *cols, double *out_hi,
                              double *out_lo) {
   (void)m;
-  for (size_t ri = 0; ri < nrows; ++ri) {
-    const double *ar = a + (size_t)rows[ri] * k;
-    for (size_t ci = 0; ci < ncols; ++ci) {
-      size_t j = (size_t)cols[ci];
-      double s = 0.0, e = 0.0;
-      for (size_t p = 0; p < k; ++p) {
-        double x = ar[p], y = b[p * n + j];
-        double pr = x * y;
-        double pe = fma(x, y, -pr);
-        double t = s + pr;
-        double z = t - s;
-        double se = (s - (t - z)) + (pr - z);
-        s = t;
-        e += se + pe;
-      }
-      double hi = s + e;
-      double lo = e - (hi - s);
-      out_hi[ri * ncols + ci] = hi;
-      out_lo[ri * ncols + ci] = lo;
-    }
-  }
+  orc_dd_task t = {0, n, k, 0, 0, ncols, a, b, rows, cols, out_hi, out_lo};
+  orc_dd_run(t, nrows);
 }
 
-/* Same for complex f64: C = A.B with four compensated real dot products. */
+/* Interleaved complex output: out[2 (r ncols + c)] = re, [.. + 1] = im. */
 void orc_dd_gemm_c64_sampled(size_t m, size_t n, size_t k, const double *a,
                              const double *b, size_t nrows,
                              const int64_t *rows, size_t ncols,
                              const int64_t *cols, double *out) {
   (void)m;
-  for (size_t ri = 0; ri < nrows; ++ri) {
-    const double *ar = a + 2 * (size_t)rows[ri] * k;
-    for (size_t ci = 0; ci < ncols; ++ci) {
-      size_t j = (size_t)cols[ci];
-      double s[2] = {0, 0}, e[2] = {0, 0};
-      for (size_t p = 0; p < k; ++p) {
-        double xr = ar[2 * p], xi = ar[2 * p + 1];
-        double yr = b[2 * (p * n + j)], yi = b[2 * (p * n + j) + 1];
-        double terms[2][2][2] = {{{xr, yr}, {-xi, yi}}, {{xr, yi}, {xi, yr}}};
-        for (int q = 0; q < 2; ++q)
-          for (int w = 0; w < 2; ++w) {
-            double x = terms[q][w][0], y = terms[q][w][1];
-            double pr = x * y, pe = fma(x, y, -pr);
-            double t = s[q] + pr, z = t - s[q];
-            double se = (s[q] - (t - z)) + (pr - z);
-            s[q] = t;
-            e[q] += se + pe;
-          }
-      }
-      out[2 * (ri * ncols + ci)] = s[0] + e[0];
-      out[2 * (ri * ncols + ci) + 1] = s[1] + e[1];
-    }
-  }
+  orc_dd_task t = {1, n, k, 0, 0, ncols, a, b, rows, cols, out, NULL};
+  orc_dd_run(t, nrows);
 }
 
 /* ------------------------------------------------------------------ */

@@ -39,13 +39,18 @@ def _gpu(lower, m, n, alpha, a, lda, b, ldb, unit):
         fa.last_status()
 
 
-@pytest.mark.parametrize("lower", [True, False])
+# Single-leaf solves whose operation order is the reference's: every lower
+# solve up to 64 rows (n >= 16384 runs the lower leaf with 4 threads per
+# column, smaller n with 8), upper solves up to 32 rows (above that the
+# reference splits at 32 while the device leaf runs on to 64 rows).
+LEAF_EXACT = ([(True, m, n) for m, n in [(1, 1), (5, 3), (32, 9), (64, 300), (47, 16400),
+                                         (64, 16500)]] +
+              [(False, m, n) for m, n in [(1, 1), (5, 3), (32, 9), (17, 16400)]])
+
+
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
-@pytest.mark.parametrize("m,n", [(1, 1), (5, 3), (32, 9), (64, 300), (47, 16400), (64, 16500)])
+@pytest.mark.parametrize("lower,m,n", LEAF_EXACT)
 def test_single_leaf_bit_exact(gpu, lower, dt, m, n):
-    # n >= 16384 runs the lower leaf with 4 threads per column, smaller n with 8
-    if not lower and m > 32:
-        pytest.skip("upper solves above 32 rows take the reference's split order")
     a, b = _system(lower, m, n, dt, 1)
     x_gpu = b.ravel().copy()
     x_orc = b.ravel().copy()
@@ -54,9 +59,16 @@ def test_single_leaf_bit_exact(gpu, lower, dt, m, n):
     assert x_gpu.tobytes() == x_orc.tobytes()
 
 
-@pytest.mark.parametrize("lower", [True, False])
+# Solves whose order differs from the reference's substitution: the blocked
+# recursion (m > 64), and upper single-leaf solves with 33 <= m <= 64, which
+# the reference splits at 32 rows and the device solves in one leaf.
+BLOCKED = ([(lower, m, n) for lower in (True, False)
+            for m, n in [(65, 4), (80, 7), (300, 50), (1000, 33), (517, 1)]] +
+           [(False, m, n) for m, n in [(33, 5), (40, 300), (64, 16500), (64, 1)]])
+
+
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
-@pytest.mark.parametrize("m,n", [(65, 4), (80, 7), (300, 50), (1000, 33), (517, 1)])
+@pytest.mark.parametrize("lower,m,n", BLOCKED)
 def test_blocked_against_oracle(gpu, lower, dt, m, n):
     a, b = _system(lower, m, n, dt, 2)
     x_gpu = b.ravel().copy()

@@ -558,3 +558,219 @@ def test_trsm_lower_recursion_keeps_the_substitution_order():
             y[i] = y[i] - a[i, p] * y[p]
         y[i] = y[i] * (1.0 / a[i, i])
     assert x.reshape(m, n).tobytes() == y.tobytes()
+
+
+# ------------------------------------------- helpers the config-size checks use
+
+def test_dd_complex_reference_meets_the_dot2_bound():
+    """orc_dd_gemm_c64_sampled against exact rational sums.  Dot2 (Ogita,
+    Rump, Oishi 2005, Prop. 5.5) returns s with |s - x.y| <= u|x.y| +
+    gamma_n^2 |x|.|y|, here per part over n = 2k products.  Inputs span
+    2^+-30 so the lo parts matter; a sign slip in either cross term (re uses
+    -xi yi, im uses +xi yr) is off by 2|xi yi| and fails."""
+    r = rng(21)
+    m, n, k = 3, 4, 150
+    sc = lambda s: np.exp2(r.integers(-30, 30, s))  # noqa: E731
+    A = (r.standard_normal((m, k)) * sc((m, k)) + 1j * r.standard_normal((m, k)) * sc((m, k)))
+    B = (r.standard_normal((k, n)) * sc((k, n)) + 1j * r.standard_normal((k, n)) * sc((k, n)))
+    out = oracle.dd_gemm_c64_sampled(m, n, k, A.view(np.float64).ravel(),
+                                     B.view(np.float64).ravel(), np.arange(m), np.arange(n))
+    u = Fraction(2) ** -53
+    g = 2 * k * u / (1 - 2 * k * u)
+    for i in range(m):
+        for j in range(n):
+            xs = [(Fraction(A[i, p].real), Fraction(A[i, p].imag)) for p in range(k)]
+            ys = [(Fraction(B[p, j].real), Fraction(B[p, j].imag)) for p in range(k)]
+            re = sum(xr * yr - xi * yi for (xr, xi), (yr, yi) in zip(xs, ys))
+            im = sum(xr * yi + xi * yr for (xr, xi), (yr, yi) in zip(xs, ys))
+            mag = sum(abs(xr) * abs(yr) + abs(xi) * abs(yi) + abs(xr) * abs(yi) + abs(xi) * abs(yr)
+                      for (xr, xi), (yr, yi) in zip(xs, ys))
+            for got, ex in ((out[i, j].real, re), (out[i, j].imag, im)):
+                assert abs(Fraction(got) - ex) <= u * abs(ex) + g * g * mag + Fraction(2) ** -1000
+
+
+def test_dd_complex_reference_on_exact_cancellation():
+    """Small integers: every sum is exact, so the result must equal numpy's
+    integer product exactly, including rows that cancel to zero."""
+    r = rng(22)
+    m, n, k = 5, 6, 64
+    A = r.integers(-7, 8, (m, k)) + 1j * r.integers(-7, 8, (m, k))
+    B = r.integers(-7, 8, (k, n)) + 1j * r.integers(-7, 8, (k, n))
+    A[2] = 1j * A[1]          # row 2 = i row 1: C[2] = i C[1]
+    out = oracle.dd_gemm_c64_sampled(m, n, k, A.view(np.float64).ravel(),
+                                     B.view(np.float64).ravel(), np.arange(m), np.arange(n))
+    assert np.array_equal(out, A @ B)
+    assert np.array_equal(out[2], 1j * out[1])
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_dd_reference_row_split_is_invisible(cplx):
+    """The rows are split over threads; any subset of rows, asked for alone,
+    gives the same bits as in one call (the arithmetic per element is one
+    sequential loop)."""
+    r = rng(23)
+    m, n, k = 37, 9, 333
+    w = 2 if cplx else 1
+    a = r.standard_normal(m * k * w); b = r.standard_normal(k * n * w)
+    fn = oracle.dd_gemm_c64_sampled if cplx else (lambda *x: oracle.dd_gemm_f64_sampled(*x)[0])
+    rows = np.arange(m); cols = np.array([0, 3, 8])
+    full = fn(m, n, k, a, b, rows, cols)
+    for i in (0, 17, 36):
+        one = fn(m, n, k, a, b, np.array([i]), cols)
+        assert one.tobytes() == full[i:i + 1].tobytes()
+
+
+@pytest.mark.parametrize("m,n,k,workers", [(1, 5, 7, 8), (3, 1, 300, 8), (17, 13, 9, 4),
+                                           (64, 33, 129, 0), (9, 4, 0, 3)])
+def test_i8_signed_threaded_is_the_closed_form(m, n, k, workers):
+    """The threaded integer product splits rows over workers (fewer rows
+    than workers, one column, k = 0 included) and equals orc_gemm_i8_signed
+    and numpy's exact product mod 2^32."""
+    r = rng(24)
+    a = r.integers(-128, 128, max(m * k, 1)).astype(np.int8)
+    b = r.integers(-128, 128, max(k * n, 1)).astype(np.int8)
+    c1 = np.full(m * n, 7, np.int32); c2 = np.zeros(m * n, np.int32)
+    oracle.gemm_i8_signed_threaded(m, n, k, a, b, c1, workers)
+    oracle.gemm_i8_signed(m, n, k, a, b, c2, False)
+    assert np.array_equal(c1, c2)
+    want = wrap_i32(exact_int_matmul(a[:m * k].reshape(m, k), b[:k * n].reshape(k, n))).ravel()
+    assert np.array_equal(c1, want)
+
+
+def test_i8_signed_threaded_wraps():
+    """k = 140000 products of -128 * -128 = 2^14 sum to 2293760000 > 2^31 - 1:
+    the output is that value mod 2^32, not a saturated i32."""
+    m, n, k = 2, 3, 140000
+    a = np.full(m * k, -128, np.int8); b = np.full(k * n, -128, np.int8)
+    c = np.zeros(m * n, np.int32)
+    oracle.gemm_i8_signed_threaded(m, n, k, a, b, c, 2)
+    assert np.all(c == np.int64(k * 2 ** 14 - 2 ** 32))
+
+
+def test_gemm_c32_shapes_alpha_beta_and_blocking():
+    """The c64 checks repeated on the f32 restatement: non-square with k past
+    one KC block, complex alpha / beta against numpy in double within the
+    f32 accumulation bound, exact small-integer products, k == 0 with
+    complex beta, and the same per-block formula as c64 (on f32-exact inputs
+    whose partial sums are exact in f32 the two agree bit for bit)."""
+    r = rng(25)
+    m, n, k = 13, 7, 300
+    A = (r.standard_normal((m, k)) + 1j * r.standard_normal((m, k))).astype(np.complex64)
+    B = (r.standard_normal((k, n)) + 1j * r.standard_normal((k, n))).astype(np.complex64)
+    C0 = (r.standard_normal((m, n)) + 1j * r.standard_normal((m, n))).astype(np.complex64)
+    al, be = np.complex64(0.5 - 2j), np.complex64(-1.5 + 0.25j)
+    c = C0.copy()
+    oracle.gemm_c32(m, n, k, al.real, al.imag, A.view(np.float32).ravel(),
+                    B.view(np.float32).ravel(), be.real, be.imag, c.view(np.float32).ravel())
+    A64, B64, C64 = A.astype(np.complex128), B.astype(np.complex128), C0.astype(np.complex128)
+    ref = complex(al) * (A64 @ B64) + complex(be) * C64
+    mag = abs(complex(al)) * (np.abs(A64.real) + np.abs(A64.imag)) @ (np.abs(B64.real) +
+                                                                     np.abs(B64.imag))
+    bound = 2 * (k + 4) * 2.0 ** -24 * (mag + abs(complex(be)) * np.abs(C64))
+    assert np.all(np.abs(c - ref) <= bound)
+    # a transposed or conjugated operand is far outside the bound
+    assert np.max(np.abs(c - (complex(al) * (A64 @ B64.conj()) + complex(be) * C64))) > 10 * bound.max()
+    # exact small-integer products
+    Ai = (r.integers(-4, 5, (m, k)) + 1j * r.integers(-4, 5, (m, k))).astype(np.complex64)
+    Bi = (r.integers(-4, 5, (k, n)) + 1j * r.integers(-4, 5, (k, n))).astype(np.complex64)
+    ci = np.zeros((m, n), np.complex64)
+    oracle.gemm_c32(m, n, k, 1.0, 0.0, Ai.view(np.float32).ravel(), Bi.view(np.float32).ravel(),
+                    0.0, 0.0, ci.view(np.float32).ravel())
+    assert np.array_equal(ci, (Ai.astype(np.complex128) @ Bi.astype(np.complex128)))
+    c64 = np.zeros((m, n), np.complex128)
+    oracle.gemm_c64(m, n, k, 1.0, 0.0, Ai.astype(np.complex128).view(np.float64).ravel(),
+                    Bi.astype(np.complex128).view(np.float64).ravel(), 0.0, 0.0,
+                    c64.view(np.float64).ravel())
+    assert np.array_equal(ci.astype(np.complex128), c64)
+    # k == 0 with complex beta (outer.rs:1502-1519)
+    c = C0.copy()
+    oracle.gemm_c32(m, n, 0, 1.0, 0.0, np.zeros(2, np.float32), np.zeros(2, np.float32),
+                    be.real, be.imag, c.view(np.float32).ravel())
+    assert np.allclose(c, complex(be) * C64, rtol=2e-7, atol=0)
+
+
+def test_gemm_c32_alpha_applies_per_block():
+    """With alpha = 1 + 0i the product is unscaled; with alpha = 3 each KC
+    block is scaled before it is added (k = 257 = one block + one element),
+    which on integers is still exact: alpha*(A B)."""
+    r = rng(26)
+    m, n, k = 4, 5, 257
+    A = (r.integers(-3, 4, (m, k)) + 1j * r.integers(-3, 4, (m, k))).astype(np.complex64)
+    B = (r.integers(-3, 4, (k, n)) + 1j * r.integers(-3, 4, (k, n))).astype(np.complex64)
+    c = np.zeros((m, n), np.complex64)
+    oracle.gemm_c32(m, n, k, 3.0, 0.0, A.view(np.float32).ravel(), B.view(np.float32).ravel(),
+                    0.0, 0.0, c.view(np.float32).ravel())
+    assert np.array_equal(c, 3 * (A.astype(np.complex128) @ B.astype(np.complex128)))
+
+
+def test_f16_gemm_is_widen_then_sgemm():
+    """gemm_f16_f32 (mod.rs:906-933) = widen every binary16 to f32, then the
+    f32 block formula; numpy's float16 -> float32 conversion does the
+    widening independently."""
+    r = rng(27)
+    m, n, k = 9, 10, 300
+    a16 = r.standard_normal(m * k).astype(np.float16)
+    b16 = r.standard_normal(k * n).astype(np.float16)
+    c = np.zeros(m * n, np.float32)
+    oracle.gemm_f16_f32(m, n, k, a16.view(np.uint16), b16.view(np.uint16), c, False)
+    aw, bw = a16.astype(np.float32), b16.astype(np.float32)
+    c2 = np.zeros(m * n, np.float32)
+    oracle.gemm_f32(m, n, k, 1.0, aw, bw, 0.0, c2)
+    assert np.array_equal(c, c2)
+    c3 = np.full(m * n, 0.25, np.float32)
+    oracle.gemm_f16_f32(m, n, k, a16.view(np.uint16), b16.view(np.uint16), c3, True)
+    c4 = np.full(m * n, 0.25, np.float32)
+    oracle.gemm_f32(m, n, k, 1.0, aw, bw, 1.0, c4)
+    assert np.array_equal(c3, c4)
+
+
+# ------------------------------------------------------------------ generators
+
+def _splitmix_py(state: int, count: int):
+    """SplitMix64 as published by Steele, Lea and Flood (2014) / Vigna's
+    splitmix64.c: state += 0x9E3779B97F4A7C15, then the two xor-shift-
+    multiply rounds and a final xor-shift."""
+    out = []
+    for _ in range(count):
+        state = (state + 0x9E3779B97F4A7C15) & (2 ** 64 - 1)
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2 ** 64 - 1)
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2 ** 64 - 1)
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def test_splitmix64_is_the_published_generator():
+    # first outputs of the generator started at state 0
+    assert _splitmix_py(0, 3) == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    for seed in (0, 1, 7, 2 ** 63 + 5):
+        want = _splitmix_py(seed, 50)
+        assert [oracle.lib().orc_splitmix64(seed, i) for i in range(50)] == want
+
+
+def test_uniform_generators():
+    """uniform_f64: the top 53 bits of each draw, r * 2^-52 - 1, so exact
+    multiples of 2^-52 in [-1, 1); uniform_int: lo + floor(r32 * span / 2^32)
+    of the draw's top 32 bits."""
+    draws = _splitmix_py(9, 4000)
+    f = oracle.uniform_f64(4000, 9)
+    assert f.tolist() == [(d >> 11) * 2.0 ** -52 - 1.0 for d in draws]
+    assert f.min() >= -1.0 and f.max() < 1.0
+    i = oracle.uniform_int(4000, 9, -128, 127)
+    assert i.tolist() == [-128 + (((d >> 32) * 256) >> 32) for d in draws]
+    assert i.min() == -128 and i.max() == 127
+    j = oracle.uniform_int(4000, 9, -32768, 32767)
+    assert j.min() < -32000 and j.max() > 32000
+
+
+def test_cpu_feature_probe_matches_the_kernel_flags():
+    flags = set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+    except OSError:
+        pytest.skip("no /proc/cpuinfo")
+    assert oracle.cpu_has_avx2_fma() == ({"avx2", "fma"} <= flags)
