@@ -90,6 +90,7 @@ struct Req {
   int state = ST_QUEUED;
   int status = FERRAY_OK;
   bool holds_c = false, holds_ab = false;
+  int readers = 0;  // ferray_server_result copying C out with the lock dropped
   size_t c_off = 0, c_bytes = 0, a_off = 0, a_bytes = 0, b_off = 0, b_bytes = 0;
   Clock::time_point t_arrive, t_done;
 };
@@ -425,12 +426,19 @@ static int submit_at(ferray_server* s, const ferray_request& r, Clock::time_poin
   return FERRAY_OK;
 }
 
-// Waits for `t` with the lock held by `lk`; returns the request or nullptr.
+// Waits, with the lock held by `lk`, until `t` is done and no result copy of
+// it is in flight; returns the request, or nullptr if the ticket is unknown
+// or was released meanwhile.  The ticket is looked up again on every wake-up:
+// a submission may rehash the map while the lock is dropped, and another
+// thread may release the ticket.
 static Req* wait_done(ferray_server* s, uint64_t t, std::unique_lock<std::mutex>& lk) {
-  auto it = s->reqs.find(t);
-  if (it == s->reqs.end()) return nullptr;
-  s->cv_done.wait(lk, [&] { return it->second.state == ST_DONE; });
-  return &it->second;
+  Req* q = nullptr;
+  s->cv_done.wait(lk, [&] {
+    auto it = s->reqs.find(t);
+    q = it == s->reqs.end() ? nullptr : &it->second;
+    return !q || (q->state == ST_DONE && q->readers == 0);
+  });
+  return q;
 }
 
 static void release_locked(ferray_server* s, uint64_t t) {
@@ -570,11 +578,17 @@ int ferray_server_result(ferray_server* s, uint64_t t, void* c_host, size_t byte
     if (q->status) return q->status;
     if (!q->holds_c || bytes != q->c_bytes) return FERRAY_ERR_INVALID;
     src = s->arena.base() + q->c_off;
+    q->readers = 1;  // a concurrent release or result waits for the copy
   }
-  FG_CHECK(hipMemcpy(c_host, src, bytes, hipMemcpyDeviceToHost));
+  const hipError_t e = hipMemcpy(c_host, src, bytes, hipMemcpyDeviceToHost);
   std::lock_guard<std::mutex> lk(s->mu);
-  release_locked(s, t);
-  return FERRAY_OK;
+  auto it = s->reqs.find(t);
+  if (it != s->reqs.end()) {
+    it->second.readers = 0;
+    if (e == hipSuccess) release_locked(s, t);
+  }
+  s->cv_done.notify_all();
+  return e == hipSuccess ? FERRAY_OK : map_hip_error(e);
 }
 
 int ferray_server_release(ferray_server* s, uint64_t t) {

@@ -444,56 +444,74 @@ typedef struct {
   double *out_hi, *out_lo;
 } orc_dd_task;
 
+/* One row at a time, all selected columns together: step p updates every
+ * column's (s, e) pair, so B is read along its rows.  Each element still sees
+ * the terms p = 0, 1, ..., k-1 in order with the same operations. */
 static void orc_dd_rows_f64(const orc_dd_task *t) {
+  double *s = (double *)malloc(2 * t->ncols * sizeof(double));
+  double *e = s + t->ncols;
   for (size_t ri = t->r0; ri < t->r1; ++ri) {
     const double *ar = t->a + (size_t)t->rows[ri] * t->k;
-    for (size_t ci = 0; ci < t->ncols; ++ci) {
-      size_t j = (size_t)t->cols[ci];
-      double s = 0.0, e = 0.0;
-      for (size_t p = 0; p < t->k; ++p) {
-        double x = ar[p], y = t->b[p * t->n + j];
+    for (size_t ci = 0; ci < t->ncols; ++ci) s[ci] = e[ci] = 0.0;
+    for (size_t p = 0; p < t->k; ++p) {
+      const double x = ar[p];
+      const double *brow = t->b + p * t->n;
+      for (size_t ci = 0; ci < t->ncols; ++ci) {
+        double y = brow[t->cols[ci]];
         double pr = x * y;
         double pe = fma(x, y, -pr);
-        double u = s + pr;
-        double z = u - s;
-        double se = (s - (u - z)) + (pr - z);
-        s = u;
-        e += se + pe;
+        double u = s[ci] + pr;
+        double z = u - s[ci];
+        double se = (s[ci] - (u - z)) + (pr - z);
+        s[ci] = u;
+        e[ci] += se + pe;
       }
-      double hi = s + e;
-      double lo = e - (hi - s);
+    }
+    for (size_t ci = 0; ci < t->ncols; ++ci) {
+      double hi = s[ci] + e[ci];
+      double lo = e[ci] - (hi - s[ci]);
       t->out_hi[ri * t->ncols + ci] = hi;
       t->out_lo[ri * t->ncols + ci] = lo;
     }
   }
+  free(s);
 }
 
 /* Complex f64: C = A.B with the real part sum(xr yr) + sum(-xi yi) and the
- * imaginary part sum(xr yi) + sum(xi yr), each one compensated sum. */
+ * imaginary part sum(xr yi) + sum(xi yr), each one compensated sum (terms in
+ * the order xr yr, -xi yi for the real part and xr yi, xi yr for the
+ * imaginary part, p increasing). */
 static void orc_dd_rows_c64(const orc_dd_task *t) {
+  double *st = (double *)malloc(4 * t->ncols * sizeof(double));
   for (size_t ri = t->r0; ri < t->r1; ++ri) {
     const double *ar = t->a + 2 * (size_t)t->rows[ri] * t->k;
-    for (size_t ci = 0; ci < t->ncols; ++ci) {
-      size_t j = (size_t)t->cols[ci];
-      double s[2] = {0, 0}, e[2] = {0, 0};
-      for (size_t p = 0; p < t->k; ++p) {
-        double xr = ar[2 * p], xi = ar[2 * p + 1];
-        double yr = t->b[2 * (p * t->n + j)], yi = t->b[2 * (p * t->n + j) + 1];
-        double terms[2][2][2] = {{{xr, yr}, {-xi, yi}}, {{xr, yi}, {xi, yr}}};
+    for (size_t ci = 0; ci < 4 * t->ncols; ++ci) st[ci] = 0.0;
+    for (size_t p = 0; p < t->k; ++p) {
+      const double xr = ar[2 * p], xi = ar[2 * p + 1];
+      const double *brow = t->b + 2 * p * t->n;
+      for (size_t ci = 0; ci < t->ncols; ++ci) {
+        const size_t j = (size_t)t->cols[ci];
+        const double yr = brow[2 * j], yi = brow[2 * j + 1];
+        const double terms[2][2][2] = {{{xr, yr}, {-xi, yi}}, {{xr, yi}, {xi, yr}}};
+        double *sq = st + 4 * ci;  /* s_re, e_re, s_im, e_im */
         for (int q = 0; q < 2; ++q)
           for (int w = 0; w < 2; ++w) {
             double x = terms[q][w][0], y = terms[q][w][1];
             double pr = x * y, pe = fma(x, y, -pr);
-            double u = s[q] + pr, z = u - s[q];
-            double se = (s[q] - (u - z)) + (pr - z);
-            s[q] = u;
-            e[q] += se + pe;
+            double u = sq[2 * q] + pr, z = u - sq[2 * q];
+            double se = (sq[2 * q] - (u - z)) + (pr - z);
+            sq[2 * q] = u;
+            sq[2 * q + 1] += se + pe;
           }
       }
-      t->out_hi[2 * (ri * t->ncols + ci)] = s[0] + e[0];
-      t->out_hi[2 * (ri * t->ncols + ci) + 1] = s[1] + e[1];
+    }
+    for (size_t ci = 0; ci < t->ncols; ++ci) {
+      const double *sq = st + 4 * ci;
+      t->out_hi[2 * (ri * t->ncols + ci)] = sq[0] + sq[1];
+      t->out_hi[2 * (ri * t->ncols + ci) + 1] = sq[2] + sq[3];
     }
   }
+  free(st);
 }
 
 static void *orc_dd_worker(void *arg) {
