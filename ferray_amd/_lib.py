@@ -47,6 +47,7 @@ SIGNATURES: dict[str, tuple] = {
     "ferray_gemm_c32_dev": (_i, [_sz, _sz, _sz, _f, _f, _p, _p, _f, _f, _p, _p]),
     "ferray_gemm_i8_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
     "ferray_gemm_i8_signed_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
+    "ferray_gemm_i8_signed_dev_noalloc": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
     "ferray_gemm_i16_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
     "ferray_quantized_matmul_dev": (_i, [_sz, _sz, _sz, _p, _u8, _f, _p, _i8, _f, _p, _b, _p]),
     "ferray_gemm_bf16_f32_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),

@@ -24,6 +24,7 @@
 // K is reduced in chunks of at most 2^16 so that no i32 accumulator can
 // leave the i32 range whatever the operands (|a*b| <= 2^14 per term).
 #include <algorithm>
+#include <mutex>
 
 #include "mfma_nt.hpp"
 
@@ -359,10 +360,17 @@ struct Carver {
 // Driver
 // ---------------------------------------------------------------------
 
+// Caller-provided scratch in place of the per-stream workspace (the
+// allocation-free entry point): `bytes` of device memory at `ptr`.
+struct ExtScratch {
+  uint8_t* ptr;
+  size_t bytes;
+};
+
 template <class S>
 static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, const void* b,
                         void* c, bool accumulate, uint8_t a_zp, float scale_a, int8_t b_zp,
-                        float scale_b, hipStream_t st) {
+                        float scale_b, hipStream_t st, const ExtScratch* ext) {
   const bool half = (kind == NT_BF16 || kind == NT_F16);
   const bool fp8 = (kind == NT_E4M3 || kind == NT_E5M2);
   const bool is_u8 = (kind == NT_U8 || kind == NT_QUANT);
@@ -390,8 +398,15 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
   const size_t o_a0 = direct_a ? 0 : cv.take(flip_a ? m * k : Mp * Kp * pe);
   const size_t o_a1 = (!direct_a && planes == 2) ? cv.take(Mp * Kp) : 0;
   void* wsv = nullptr;
-  int s = workspace(st, cv.off, &wsv);
-  if (s) return s;
+  if (ext) {
+    // Checked before anything is enqueued: an oversized problem leaves C
+    // untouched.
+    if (cv.off > ext->bytes) return FERRAY_ERR_SCRATCH;
+    wsv = ext->ptr;
+  } else {
+    int s = workspace(st, cv.off, &wsv);
+    if (s) return s;
+  }
   uint8_t* ws = reinterpret_cast<uint8_t*>(wsv);
   int32_t* csp = need_colsum ? reinterpret_cast<int32_t*>(ws + o_cs) : nullptr;
   int32_t* rsp = need_rowsum ? reinterpret_cast<int32_t*>(ws + o_rs) : nullptr;
@@ -553,7 +568,7 @@ __global__ void quant_finish_kernel(const int32_t* __restrict__ raw, float* c, i
 
 static int run_nt(int kind, size_t m, size_t n, size_t k, const void* a, const void* b, void* c,
                   bool accumulate, uint8_t a_zp, float scale_a, int8_t b_zp, float scale_b,
-                  hipStream_t st);
+                  hipStream_t st, const ExtScratch* ext = nullptr);
 
 // quantized_matmul with k > 2^16: the exact wrapping raw product is formed in
 // a stream-ordered temporary by the chunked u8 path, then finished
@@ -585,7 +600,7 @@ static int run_quant_long(size_t m, size_t n, size_t k, const void* a, const voi
 
 static int run_nt(int kind, size_t m, size_t n, size_t k, const void* a, const void* b, void* c,
                   bool accumulate, uint8_t a_zp, float scale_a, int8_t b_zp, float scale_b,
-                  hipStream_t st) {
+                  hipStream_t st, const ExtScratch* ext) {
   if (m == 0 || n == 0) return FERRAY_OK;
   if (!c || (k && (!a || !b))) return FERRAY_ERR_INVALID;
   if (m > (1u << 30) || n > (1u << 30) || k > (1u << 30)) return FERRAY_ERR_INVALID;
@@ -597,6 +612,7 @@ static int run_nt(int kind, size_t m, size_t n, size_t k, const void* a, const v
     }
     return FERRAY_OK;
   }
+  if (ext && kind == NT_QUANT) return FERRAY_ERR_INVALID;  // the long form allocates
   if (kind == NT_QUANT && k > NT_K_CHUNK) return run_quant_long(m, n, k, a, b, c, accumulate,
                                                                 a_zp, scale_a, b_zp, scale_b, st);
   // 256 x 256 tiles when they give at least 128 blocks; below that the
@@ -605,9 +621,40 @@ static int run_nt(int kind, size_t m, size_t n, size_t k, const void* a, const v
   // Both shapes reduce k in the same order, so results are identical.
   if (m >= 256 && n >= 256 && cdiv(m, (size_t)256) * cdiv(n, (size_t)256) >= 128)
     return run_nt_shape<NtBig>(kind, m, n, k, a, b, c, accumulate, a_zp, scale_a, b_zp, scale_b,
-                               st);
+                               st, ext);
   return run_nt_shape<NtSmall>(kind, m, n, k, a, b, c, accumulate, a_zp, scale_a, b_zp, scale_b,
-                               st);
+                               st, ext);
+}
+
+// ---------------------------------------------------------------------
+// Fixed scratch of the allocation-free i8 entry point
+// ---------------------------------------------------------------------
+// One array per device, allocated with the code object when the library is
+// loaded and held until the process ends.  In host code `noalloc_scratch`
+// names a host-side stand-in; the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)128 << 20;
+static __device__ __align__(256) uint8_t noalloc_scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static uint8_t* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+static int noalloc_scratch_of_current_device(ExtScratch* out) {
+  int dev = -1;
+  FG_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return FERRAY_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+    if (e != hipSuccess) return map_hip_error(e);
+    if (!p) return FERRAY_ERR_HIP;
+    g_noalloc_ptr[dev] = reinterpret_cast<uint8_t*>(p);
+  }
+  out->ptr = g_noalloc_ptr[dev];
+  out->bytes = NOALLOC_SCRATCH_BYTES;
+  return FERRAY_OK;
 }
 
 }  // namespace fg
@@ -628,6 +675,18 @@ int ferray_gemm_i8_signed_dev(size_t m, size_t n, size_t k, const int8_t* a, con
   int s = ensure_device();
   if (s) return s;
   return run_nt(NT_I8S, m, n, k, a, b, c, accumulate, 0, 1.f, 0, 1.f, resolve_stream(stream));
+}
+
+int ferray_gemm_i8_signed_dev_noalloc(size_t m, size_t n, size_t k, const int8_t* a,
+                                      const int8_t* b, int32_t* c, bool accumulate,
+                                      void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  if (!stream) return FERRAY_ERR_INVALID;
+  ExtScratch ext;
+  if ((s = noalloc_scratch_of_current_device(&ext))) return s;
+  return run_nt(NT_I8S, m, n, k, a, b, c, accumulate, 0, 1.f, 0, 1.f,
+                reinterpret_cast<hipStream_t>(stream), &ext);
 }
 
 int ferray_gemm_i16_dev(size_t m, size_t n, size_t k, const int16_t* a, const int16_t* b,

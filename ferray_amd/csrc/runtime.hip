@@ -197,6 +197,7 @@ const char* ferray_status_string(int status) {
     case FERRAY_ERR_INVALID: return "invalid argument";
     case FERRAY_ERR_HIP: return "HIP runtime error";
     case FERRAY_ERR_UNSUPPORTED: return "unsupported configuration";
+    case FERRAY_ERR_SCRATCH: return "problem exceeds the fixed scratch of the allocation-free entry";
     default: return "unknown status";
   }
 }

@@ -34,6 +34,7 @@ extern "C" {
 #define FERRAY_ERR_INVALID 3
 #define FERRAY_ERR_HIP 4
 #define FERRAY_ERR_UNSUPPORTED 5
+#define FERRAY_ERR_SCRATCH 6 /* problem needs more than a fixed scratch holds */
 
 /* ---------------- drop-in entry points (host pointers) ---------------- */
 
@@ -115,6 +116,26 @@ int ferray_gemm_i8_dev(size_t m, size_t n, size_t k, const uint8_t *a,
 int ferray_gemm_i8_signed_dev(size_t m, size_t n, size_t k, const int8_t *a,
                               const int8_t *b, int32_t *c, bool accumulate,
                               void *stream);
+/* Allocation-free form of ferray_gemm_i8_signed_dev, with the same
+ * arguments, semantics and bits.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * hipMalloc / hipFree; the packed B^T planes (and a packed copy of A when A
+ * is not a whole number of tiles) live in one fixed 128 MiB device array per
+ * GPU, held until the process ends.  A problem needs
+ *   Np*Kp + 4*(Mp + Np) (+ Mp*Kp when A is packed) bytes, rounded up to 256
+ * each, where Mp, Np round m, n up to the tile (256 when the product has at
+ * least 128 tiles of 256 x 256, else 128) and Kp rounds k up to 128 (256-row
+ * tiles) or 64; A is used in place when m is a whole number of tiles, k a
+ * multiple of the k step and A 16-byte aligned.  Larger problems return
+ * FERRAY_ERR_SCRATCH before anything is enqueued.  `stream` must not be NULL.
+ * Every call on one GPU shares that GPU's array: calls that overlap in time
+ * on one GPU (different streams or threads) race and must be serialised by
+ * the caller. */
+int ferray_gemm_i8_signed_dev_noalloc(size_t m, size_t n, size_t k,
+                                      const int8_t *a, const int8_t *b,
+                                      int32_t *c, bool accumulate,
+                                      void *stream);
 int ferray_gemm_i16_dev(size_t m, size_t n, size_t k, const int16_t *a,
                         const int16_t *b, int32_t *c, bool accumulate,
                         void *stream);
