@@ -4,7 +4,8 @@
 //   i8 x i8 -> i32    one i8 pass                              (gemm_i8_signed)
 //   u8 x i8 -> i32    A's sign bit flipped on load (a - 128), plus 128*colsum(B)
 //                                                                      (gemm_i8)
-//   i16 x i16 -> i32  four i8 byte-plane products in one launch       (gemm_i16)
+//   i16 x i16 -> i32  the four i8 byte-plane products of a tile in one block
+//                                                                     (gemm_i16)
 //   quantized_matmul  the u8 x i8 pass with a fused zero-point / scale epilogue
 //   bf16/f16 -> f32   one half-precision pass                 (gemm_{bf16,f16}_f32)
 //   fp8 -> f32        one pass on the 64-deep f8f6f4 instruction, OCP E4M3 or
@@ -13,7 +14,8 @@
 //
 // B is transposed into k-contiguous planes by a prepass (which also forms the
 // i16 byte planes and the wrapping column sums); A is read in place when its
-// shape allows, otherwise packed the same way.  Padding is zero.
+// shape allows, otherwise packed the same way (i16 A is always split into its
+// two byte planes).  Padding is zero.
 //
 // Integer arithmetic is exact mod 2^32:
 //   u8:  sum a*b = sum (a-128)*b + 128*sum_k b[k][j]
@@ -82,30 +84,6 @@ struct EpiI32 {
       if (colsum) f(EpiI32Fast<false, true>{c, N, colsum});
       else f(EpiI32Fast<false, false>{c, N, colsum});
     }
-  }
-};
-
-// The four i16 byte-plane products of one K chunk in one launch (grid row z
-// = A plane z / 2 x B plane z % 2, shifted left by 16, 8, 8, 0): each adds
-// its shifted value into C with an atomic add.  Integer addition mod 2^32 is
-// order-independent, so the result is exact whatever order the adds land
-// in.  The first chunk's row z = 0 also adds the sum corrections.
-struct EpiI16Atomic {
-  int32_t* c;
-  int M, N;
-  const int32_t* colsum;   // 128*colsum(B) term, or null
-  const int32_t* rowsum;   // 128*rowsum(A) term, or null
-  uint32_t kconst;         // -16384*k, first chunk only
-  __device__ __forceinline__ void operator()(int row, int col, int z, int acc) const {
-    if (row >= M || col >= N) return;
-    const int shift = z == 0 ? 16 : (z == 3 ? 0 : 8);
-    uint32_t v = (uint32_t)acc << shift;
-    if (z == 0) {
-      if (colsum) v += 128u * (uint32_t)colsum[col];
-      if (rowsum) v += 128u * (uint32_t)rowsum[row];
-      v += kconst;
-    }
-    atomicAdd(reinterpret_cast<unsigned int*>(c + (size_t)row * N + col), v);
   }
 };
 
@@ -261,8 +239,8 @@ __global__ __launch_bounds__(256) void pack_bt_kernel(const E* __restrict__ b, i
   }
 }
 
-// Ap planes (Mp x Kp) from row-major A (M x K), same maps; MAP 3 flips the
-// sign bit of u8 (padding stays 0).
+// Ap planes (Mp x Kp) from row-major A (M x K): MAP 0 and 2 as above; MAP 3
+// flips the sign bit of u8 (padding stays 0).
 template <typename E, int MAP>
 __global__ void pack_a_kernel(const E* __restrict__ a, int M, int K, int Mp, int Kp,
                               uint8_t* out0, uint8_t* out1) {
@@ -272,12 +250,9 @@ __global__ void pack_a_kernel(const E* __restrict__ a, int M, int K, int Mp, int
     const int m = (int)(i / Kp), k = (int)(i % Kp);
     const bool real = m < M && k < K;
     const E v = real ? a[(size_t)m * K + k] : E(0);
+    static_assert(MAP == 0 || MAP == 2 || MAP == 3, "A maps");
     if (MAP == 0) {
       out0[i] = (uint8_t)v;
-    } else if (MAP == 1) {
-      const int x = (int)v;
-      out0[i] = real ? (uint8_t)(int8_t)(x >> 8) : 0;
-      out1[i] = real ? (uint8_t)(int8_t)((x & 255) - 128) : 0;
     } else if (MAP == 2) {
       reinterpret_cast<uint16_t*>(out0)[i] = (uint16_t)v;
     } else {
@@ -357,6 +332,272 @@ struct Carver {
 };
 
 // ---------------------------------------------------------------------
+// i16 x i16 -> i32: the four byte-plane products in one block
+// ---------------------------------------------------------------------
+// A 128 x 128 tile of C per block, four waves of 64 x 64.  A stage holds 128
+// bytes of k of four planes: A_hi and A_lo' rows of the tile (the A panel,
+// LDS rows 0-127 and 128-255) and B_hi and B_lo' columns (the B panel, rows
+// 256-383 and 384-511): the same 64 KiB layout, swizzle and direct-to-LDS
+// copies as the 256 x 256 kernel, so the LDS bytes per matrix-core
+// instruction are the same too.  Each k-step multiplies every A fragment of
+// the wave with every B fragment into three register sets:
+//   HH += Ah Bh,   X += Ah Bl' + Al' Bh,   LL += Al' Bl'
+// (v_mfma_i32_16x16x64_i8, 4 x 16 tiles per set, 192 accumulator registers),
+// and the epilogue forms, wrapping in uint32,
+//   C = 2^16 HH + 2^8 X + LL + 128 rowsum(A) + 128 colsum(B) - 16384 k (+ C).
+// K is reduced in chunks of 2^15 bytes so that no accumulator (X sums two
+// products: |X| <= 2 * 2^15 * 2^14 = 2^30) leaves the i32 range.
+constexpr size_t I16_K_CHUNK = (size_t)1 << 15;
+
+struct I16Operands {
+  const uint8_t* a_hi;
+  const uint8_t* a_lo;
+  const uint8_t* b_hi;  // B^T planes, n x Kp
+  const uint8_t* b_lo;
+  size_t lda, ldb;      // row pitch (Kp) in bytes
+  int kbytes;           // bytes of k in this chunk, a multiple of 128
+  int tiles_m, tiles_n;
+  int M, N;
+  int32_t* c;
+  const int32_t* rowsum;  // first chunk: wrapping row sums of A, else null
+  const int32_t* colsum;  // first chunk: wrapping column sums of B, else null
+  uint32_t kconst;        // first chunk: -16384 * k, else 0
+  int read_c;             // add onto C
+};
+
+template <bool READ_C, bool FIRST, bool CHECK>
+__device__ __forceinline__ void i16_store(const I16Operands& p, int row, int col, int hh, int x,
+                                          int ll) {
+  if (CHECK && (row >= p.M || col >= p.N)) return;
+  uint32_t v = ((uint32_t)nt_vgpr(hh) << 16) + ((uint32_t)nt_vgpr(x) << 8) + (uint32_t)nt_vgpr(ll);
+  if (FIRST) v += 128u * ((uint32_t)p.rowsum[row] + (uint32_t)p.colsum[col]) + p.kconst;
+  int32_t* dst = p.c + (size_t)row * p.N + col;
+  if (READ_C) v += (uint32_t)*dst;
+  *dst = (int32_t)v;
+}
+
+__global__ __launch_bounds__(256) void mfma_i16_dma_kernel(I16Operands p) {
+  constexpr int KB = 128, STAGE = 512 * KB, TI = 4, TJ = 4, KSTEP = 64, NKK = KB / KSTEP;
+  constexpr int NMF = 4 * TI * TJ, NRD = 2 * (TI + TJ), NLD = 16;
+  extern __shared__ __attribute__((aligned(16))) uint8_t nt_smem[];
+
+  int tm, tn;
+  nt_tile(p.tiles_m, p.tiles_n, tm, tn);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  // Copy instruction q of wave w fills LDS rows 8 (4q + w) .. + 7 of each
+  // panel (as in the 256 x 256 kernel); q < 4 takes them from the hi plane,
+  // q >= 4 from the lo plane, at tile row 8 (4 (q & 3) + w) + lane / 8.
+  const int lrow = lane >> 3, lslot = lane & 7;
+  const int gchunk = lslot ^ ((4 * (wave & 1) + (lrow >> 1)) ^ ((lrow & 1) << 2));
+  const size_t a_t = (size_t)tm * 128 * p.lda, b_t = (size_t)tn * 128 * p.ldb;
+  const uint8_t* ga0 = p.a_hi + a_t + (size_t)(wave * 8 + lrow) * p.lda + gchunk * 16;
+  const uint8_t* ga1 = p.a_lo + a_t + (size_t)(wave * 8 + lrow) * p.lda + gchunk * 16;
+  const uint8_t* gb0 = p.b_hi + b_t + (size_t)(wave * 8 + lrow) * p.ldb + gchunk * 16;
+  const uint8_t* gb1 = p.b_lo + b_t + (size_t)(wave * 8 + lrow) * p.ldb + gchunk * 16;
+  const size_t a_q = 32 * p.lda, b_q = 32 * p.ldb;
+  auto issue = [&](int buf, int kb) __attribute__((always_inline)) {
+    uint8_t* as = nt_smem + buf * STAGE + wave * 1024;
+    uint8_t* bs = as + 256 * KB;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      __builtin_amdgcn_global_load_lds((nt_gvoid*)((q < 4 ? ga0 : ga1) + kb + (q & 3) * a_q),
+                                       (nt_lvoid*)(as + q * 4096), 16, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      __builtin_amdgcn_global_load_lds((nt_gvoid*)((q < 4 ? gb0 : gb1) + kb + (q & 3) * b_q),
+                                       (nt_lvoid*)(bs + q * 4096), 16, 0, 0);
+  };
+
+  // Fragment of k-step kk: lane l reads row l % 16 of the tile, chunk
+  // kk * 4 + l / 16 at its swizzled slot (every fragment's rows start at a
+  // multiple of 16, so the row's swizzle is that of l % 16).
+  const int swz = ((lr >> 1) & 7) ^ ((lr & 1) << 2);
+  int xo[NKK];
+#pragma unroll
+  for (int kk = 0; kk < NKK; ++kk) xo[kk] = ((kk * 4 + lk) ^ swz) << 4;
+  const int ahi = (wm0 + lr) * KB, alo = (128 + wm0 + lr) * KB;
+  const int bhi = (256 + wn0 + lr) * KB, blo = (384 + wn0 + lr) * KB;
+  // fragment sets: f[0..3] hi, f[4..7] lo
+  auto read_step = [&](const uint8_t* s, int kk, i4* fa, i4* fb) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < TI; ++i) {
+      fa[i] = *reinterpret_cast<const i4*>(s + ahi + i * 16 * KB + xo[kk]);
+      fa[TI + i] = *reinterpret_cast<const i4*>(s + alo + i * 16 * KB + xo[kk]);
+    }
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+      fb[j] = *reinterpret_cast<const i4*>(s + bhi + j * 16 * KB + xo[kk]);
+      fb[TJ + j] = *reinterpret_cast<const i4*>(s + blo + j * 16 * KB + xo[kk]);
+    }
+  };
+
+  i4 hh[TI][TJ], xx[TI][TJ], ll[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) hh[i][j] = xx[i][j] = ll[i][j] = i4{};
+  // The two updates of X are 32 instructions apart.
+  auto mma = [&](const i4* fa, const i4* fb) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        hh[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], hh[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        xx[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[TJ + j], xx[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        ll[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[TI + i], fb[TJ + j], ll[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        xx[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[TI + i], fb[j], xx[i][j], 0, 0, 0);
+  };
+
+  i4 pa[2 * TI], pb[2 * TJ];  // last k-step of the previous stage
+  // The same rotated stage body as the 256 x 256 kernel: the previous
+  // stage's last k-step is multiplied after the opening barrier, with the
+  // next stage's copies between its MFMAs.
+  auto body = [&](auto pend_c, auto more_c, int cur, int kb_next) __attribute__((always_inline)) {
+    constexpr bool PEND = decltype(pend_c)::value, MORE = decltype(more_c)::value;
+    const uint8_t* s = nt_smem + cur * STAGE;
+    i4 fa[2 * TI], fb[2 * TJ];
+    read_step(s, 0, fa, fb);
+    if (MORE) issue(cur ^ 1, kb_next);
+    if (PEND) mma(pa, pb);
+    read_step(s, 1, pa, pb);
+    mma(fa, fb);
+    __builtin_amdgcn_sched_group_barrier(0x100, NRD, 0);
+    if (PEND) {
+#pragma unroll
+      for (int q = 0; q < NMF; ++q) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (MORE && q % (NMF / NLD) == 0) __builtin_amdgcn_sched_group_barrier(0x010, 1, 0);
+      }
+    } else if (MORE) {
+      __builtin_amdgcn_sched_group_barrier(0x010, NLD, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < NRD; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x008, NMF / NRD, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+  };
+  auto barrier = [&]() __attribute__((always_inline)) {
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  using T = std::integral_constant<bool, true>;
+  using F = std::integral_constant<bool, false>;
+  static_assert(NKK == 2, "two k-steps per stage");
+
+  const int nk = p.kbytes / KB;
+  issue(0, 0);
+  __syncthreads();
+  if (nk > 1)
+    body(F{}, T{}, 0, KB);
+  else
+    body(F{}, F{}, 0, 0);
+  int kt = 1;
+  for (; kt + 1 < nk; ++kt) {
+    barrier();
+    body(T{}, T{}, kt & 1, (kt + 1) * KB);
+  }
+  if (kt < nk) {
+    barrier();
+    body(T{}, F{}, kt & 1, 0);
+  }
+  mma(pa, pb);
+
+  // Result element r of lane l of a 16 x 16 tile: row 4 (l / 16) + r,
+  // column l % 16.
+  const int row0 = tm * 128 + wm0, col0 = tn * 128 + wn0;
+  auto store_all = [&](auto rc, auto first, auto check) __attribute__((always_inline)) {
+    constexpr bool RC = decltype(rc)::value, FI = decltype(first)::value,
+                   CK = decltype(check)::value;
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          i16_store<RC, FI, CK>(p, row0 + i * 16 + 4 * lk + r, col0 + j * 16 + lr, hh[i][j][r],
+                                xx[i][j][r], ll[i][j][r]);
+  };
+  const bool inside = tm * 128 + 128 <= p.M && tn * 128 + 128 <= p.N;
+  const bool first = p.rowsum != nullptr;
+  if (inside) {
+    if (p.read_c) {
+      if (first) store_all(T{}, T{}, F{}); else store_all(T{}, F{}, F{});
+    } else {
+      if (first) store_all(F{}, T{}, F{}); else store_all(F{}, F{}, F{});
+    }
+  } else {
+    if (p.read_c) {
+      if (first) store_all(T{}, T{}, T{}); else store_all(T{}, F{}, T{});
+    } else {
+      if (first) store_all(F{}, T{}, T{}); else store_all(F{}, F{}, T{});
+    }
+  }
+}
+
+// i16 rows -> hi byte plane (a >> 8) and lo' plane ((a & 255) - 128), k
+// padded with zeros to Kp; one block per row of the Mp-row planes (rows past
+// M are zero).  With `rowsum`, also the wrapping sum of each row's values.
+__global__ __launch_bounds__(256) void split_i16_rows_kernel(const int16_t* __restrict__ a, int M,
+                                                             int K, int Kp, uint8_t* hi,
+                                                             uint8_t* lo, int32_t* rowsum) {
+  __shared__ uint32_t part[256];
+  const int row = blockIdx.x;
+  const int16_t* src = a + (size_t)row * K;
+  uint8_t* dh = hi + (size_t)row * Kp;
+  uint8_t* dl = lo + (size_t)row * Kp;
+  const bool vec = ((uintptr_t)src % 16) == 0;
+  uint32_t s = 0;
+  for (int k0 = threadIdx.x * 16; k0 < Kp; k0 += blockDim.x * 16) {
+    int16_t v[16];
+    if (row < M && vec && k0 + 16 <= K) {
+      *reinterpret_cast<u4v*>(&v[0]) = *reinterpret_cast<const u4v*>(src + k0);
+      *reinterpret_cast<u4v*>(&v[8]) = *reinterpret_cast<const u4v*>(src + k0 + 8);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = (row < M && k0 + e < K) ? src[k0 + e] : int16_t(0);
+    }
+    uint8_t h[16], l[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const bool real = row < M && k0 + e < K;
+      const int x = v[e];
+      s += (uint32_t)x;
+      h[e] = real ? (uint8_t)(int8_t)(x >> 8) : 0;
+      l[e] = real ? (uint8_t)(int8_t)((x & 255) - 128) : 0;
+    }
+    *reinterpret_cast<u4v*>(dh + k0) = __builtin_bit_cast(u4v, h);
+    *reinterpret_cast<u4v*>(dl + k0) = __builtin_bit_cast(u4v, l);
+  }
+  if (!rowsum || row >= M) return;
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rowsum[row] = (int32_t)part[0];
+}
+
+// ---------------------------------------------------------------------
 // Driver
 // ---------------------------------------------------------------------
 
@@ -374,15 +615,14 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
   const bool half = (kind == NT_BF16 || kind == NT_F16);
   const bool fp8 = (kind == NT_E4M3 || kind == NT_E5M2);
   const bool is_u8 = (kind == NT_U8 || kind == NT_QUANT);
-  const size_t es = (half || kind == NT_I16) ? 2 : 1;  // input element bytes
+  const size_t es = half ? 2 : 1;                       // input element bytes
   const size_t pe = half ? 2 : 1;                       // plane element bytes
-  const int planes = (kind == NT_I16) ? 2 : 1;
   const size_t kel = S::KB / pe;                        // k elements per stage
   const size_t Kp = align_up(k, kel);
   const size_t Mp = align_up(m, S::BM), Np = align_up(n, S::BN);
-  const bool need_colsum = (kind == NT_U8 || kind == NT_QUANT || kind == NT_I16);
-  const bool need_rowsum = (kind == NT_QUANT || kind == NT_I16);
-  const bool aligned_a = (kind != NT_I16) && (m % S::BM == 0) && ((k * es) % S::KB == 0) &&
+  const bool need_colsum = (kind == NT_U8 || kind == NT_QUANT);
+  const bool need_rowsum = kind == NT_QUANT;
+  const bool aligned_a = (m % S::BM == 0) && ((k * es) % S::KB == 0) &&
                          ((uintptr_t)a % 16 == 0);
   // Tile-aligned A is read in place, except u8 A on the 256 x 256 shape:
   // there a flipped copy lets the product run the direct-to-LDS kernel,
@@ -392,11 +632,9 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
 
   Carver cv;
   const size_t o_bt0 = cv.take(Np * Kp * pe);
-  const size_t o_bt1 = planes == 2 ? cv.take(Np * Kp) : 0;
   const size_t o_cs = cv.take(Np * 4);
   const size_t o_rs = cv.take(Mp * 4);
   const size_t o_a0 = direct_a ? 0 : cv.take(flip_a ? m * k : Mp * Kp * pe);
-  const size_t o_a1 = (!direct_a && planes == 2) ? cv.take(Mp * Kp) : 0;
   void* wsv = nullptr;
   if (ext) {
     // Checked before anything is enqueued: an oversized problem leaves C
@@ -416,11 +654,7 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
                        reinterpret_cast<uint32_t*>(csp), Np);
   }
   const dim3 tgrid((unsigned)(Np / 64), (unsigned)cdiv(Kp, 64));
-  if (kind == NT_I16)
-    hipLaunchKernelGGL((pack_bt_kernel<int16_t, 1>), tgrid, dim3(256), 0, st,
-                       reinterpret_cast<const int16_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
-                       ws + o_bt0, ws + o_bt1, csp);
-  else if (half)
+  if (half)
     hipLaunchKernelGGL((pack_bt_kernel<uint16_t, 2>), tgrid, dim3(256), 0, st,
                        reinterpret_cast<const uint16_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
                        ws + o_bt0, nullptr, nullptr);
@@ -431,12 +665,8 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
   FG_CHECK(hipGetLastError());
 
   if (need_rowsum && !flip_a) {
-    if (kind == NT_I16)
-      hipLaunchKernelGGL(rowsum_kernel<int16_t>, dim3((unsigned)m), dim3(256), 0, st,
-                         reinterpret_cast<const int16_t*>(a), (int)k, rsp);
-    else
-      hipLaunchKernelGGL(rowsum_kernel<uint8_t>, dim3((unsigned)m), dim3(256), 0, st,
-                         reinterpret_cast<const uint8_t*>(a), (int)k, rsp);
+    hipLaunchKernelGGL(rowsum_kernel<uint8_t>, dim3((unsigned)m), dim3(256), 0, st,
+                       reinterpret_cast<const uint8_t*>(a), (int)k, rsp);
     FG_CHECK(hipGetLastError());
   }
 
@@ -461,11 +691,7 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
     p.lda = k;
   } else {
     const unsigned g = grid_for(Mp * Kp);
-    if (kind == NT_I16)
-      hipLaunchKernelGGL((pack_a_kernel<int16_t, 1>), dim3(g), dim3(256), 0, st,
-                         reinterpret_cast<const int16_t*>(a), (int)m, (int)k, (int)Mp, (int)Kp,
-                         ws + o_a0, ws + o_a1);
-    else if (half)
+    if (half)
       hipLaunchKernelGGL((pack_a_kernel<uint16_t, 2>), dim3(g), dim3(256), 0, st,
                          reinterpret_cast<const uint16_t*>(a), (int)m, (int)k, (int)Mp, (int)Kp,
                          ws + o_a0, nullptr);
@@ -513,24 +739,6 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
       } else {
         rc = FERRAY_ERR_INVALID;  // unreachable: run_nt routes k > 2^16 to run_quant_long
       }
-    } else if (kind == NT_I16) {
-      NtOperands r = q;
-      r.a_plane = (size_t)((ws + o_a1) - (ws + o_a0));
-      r.b_plane = (size_t)((ws + o_bt1) - (ws + o_bt0));
-      const int32_t* cs = first ? csp : nullptr;
-      const int32_t* rs = first ? rsp : nullptr;
-      const uint32_t kc = first ? (uint32_t)(0u - 16384u * (uint32_t)k) : 0u;
-      // All four byte-plane products in one launch (hi*hi<<16, hi*lo'<<8,
-      // lo'*hi<<8, lo'*lo'), adding into C; C starts at zero unless the
-      // call accumulates.
-      if (first && !accumulate) {
-        hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(m * n)), dim3(256), 0, st,
-                           reinterpret_cast<uint32_t*>(c), m * n);
-        FG_CHECK(hipGetLastError());
-      }
-      r.cross = 1;
-      EpiI16Atomic e{reinterpret_cast<int32_t*>(c), M, N, cs, rs, kc};
-      rc = launch_mfma_nt<ELEM_I8, S, false>(r, 4, e, st);
     } else {
       EpiI32 e{reinterpret_cast<int32_t*>(c), M, N, read_c,
                (kind == NT_U8 && first) ? csp : nullptr};
@@ -538,6 +746,70 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
                  : launch_mfma_nt<ELEM_I8, S, false>(q, 1, e, st);
     }
     if (rc) return rc;
+  }
+  return FERRAY_OK;
+}
+
+// i16 x i16 -> i32: split A into its two byte planes (with the row sums in
+// the same pass), transpose B into two B^T planes (with the column sums),
+// then one launch per K chunk of the four-product kernel.
+static int run_i16(size_t m, size_t n, size_t k, const int16_t* a, const int16_t* b, int32_t* c,
+                   bool accumulate, hipStream_t st, const ExtScratch* ext) {
+  const size_t Mp = align_up(m, 128), Np = align_up(n, 128), Kp = align_up(k, 128);
+  Carver cv;
+  const size_t o_a0 = cv.take(Mp * Kp), o_a1 = cv.take(Mp * Kp);
+  const size_t o_b0 = cv.take(Np * Kp), o_b1 = cv.take(Np * Kp);
+  const size_t o_cs = cv.take(Np * 4), o_rs = cv.take(Mp * 4);
+  void* wsv = nullptr;
+  if (ext) {
+    if (cv.off > ext->bytes) return FERRAY_ERR_SCRATCH;
+    wsv = ext->ptr;
+  } else {
+    int s = workspace(st, cv.off, &wsv);
+    if (s) return s;
+  }
+  uint8_t* ws = reinterpret_cast<uint8_t*>(wsv);
+  int32_t* csp = reinterpret_cast<int32_t*>(ws + o_cs);
+  int32_t* rsp = reinterpret_cast<int32_t*>(ws + o_rs);
+  hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(Np)), dim3(256), 0, st,
+                     reinterpret_cast<uint32_t*>(csp), Np);
+  hipLaunchKernelGGL((pack_bt_kernel<int16_t, 1>), dim3((unsigned)(Np / 64), (unsigned)(Kp / 64)),
+                     dim3(256), 0, st, b, (int)k, (int)n, (int)Kp, (int)Np, ws + o_b0, ws + o_b1,
+                     csp);
+  hipLaunchKernelGGL(split_i16_rows_kernel, dim3((unsigned)Mp), dim3(256), 0, st, a, (int)m,
+                     (int)k, (int)Kp, ws + o_a0, ws + o_a1, rsp);
+  FG_CHECK(hipGetLastError());
+
+  constexpr int lds = 2 * 512 * 128;
+  static bool attr_set = false;  // every caller sets the same value; a race is benign
+  if (!attr_set) {
+    FG_CHECK(hipFuncSetAttribute((const void*)mfma_i16_dma_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    attr_set = true;
+  }
+  const size_t tiles = (Mp / 128) * (Np / 128);
+  if (tiles > 0x7fffffff) return FERRAY_ERR_INVALID;
+  for (size_t kb0 = 0; kb0 < Kp; kb0 += I16_K_CHUNK) {
+    const bool first = kb0 == 0;
+    I16Operands q;
+    q.a_hi = ws + o_a0 + kb0;
+    q.a_lo = ws + o_a1 + kb0;
+    q.b_hi = ws + o_b0 + kb0;
+    q.b_lo = ws + o_b1 + kb0;
+    q.lda = Kp;
+    q.ldb = Kp;
+    q.kbytes = (int)std::min(I16_K_CHUNK, Kp - kb0);
+    q.tiles_m = (int)(Mp / 128);
+    q.tiles_n = (int)(Np / 128);
+    q.M = (int)m;
+    q.N = (int)n;
+    q.c = c;
+    q.rowsum = first ? rsp : nullptr;
+    q.colsum = first ? csp : nullptr;
+    q.kconst = first ? (uint32_t)(0u - 16384u * (uint32_t)k) : 0u;
+    q.read_c = (!first || accumulate) ? 1 : 0;
+    hipLaunchKernelGGL(mfma_i16_dma_kernel, dim3((unsigned)tiles), dim3(256), lds, st, q);
+    FG_CHECK(hipGetLastError());
   }
   return FERRAY_OK;
 }
@@ -613,6 +885,10 @@ static int run_nt(int kind, size_t m, size_t n, size_t k, const void* a, const v
     return FERRAY_OK;
   }
   if (ext && kind == NT_QUANT) return FERRAY_ERR_INVALID;  // the long form allocates
+  if (kind == NT_I16)
+    return run_i16(m, n, k, reinterpret_cast<const int16_t*>(a),
+                   reinterpret_cast<const int16_t*>(b), reinterpret_cast<int32_t*>(c), accumulate,
+                   st, ext);
   if (kind == NT_QUANT && k > NT_K_CHUNK) return run_quant_long(m, n, k, a, b, c, accumulate,
                                                                 a_zp, scale_a, b_zp, scale_b, st);
   // 256 x 256 tiles when they give at least 128 blocks; below that the

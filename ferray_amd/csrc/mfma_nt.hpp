@@ -54,9 +54,7 @@ struct NtOperands {
   size_t a_plane, b_plane;  // plane strides in bytes
   size_t lda, ldb;          // row pitch in bytes
   int kbytes;               // bytes of k to reduce, a multiple of the stage depth
-  int tiles_m, tiles_n;
-  int cross;                // 0: grid row z pairs A plane z with B plane z;
-                            // 1: it pairs A plane z / 2 with B plane z % 2
+  int tiles_m, tiles_n;      // grid row z pairs A plane z with B plane z
 };
 
 // One 32-byte k group per step: a 16-byte fragment per lane.
@@ -181,9 +179,8 @@ __global__ __launch_bounds__(S::THREADS) void mfma_nt_kernel(NtOperands p, Epi e
   int tm, tn;
   nt_tile(p.tiles_m, p.tiles_n, tm, tn);
   const int z = blockIdx.y;
-  const int za = p.cross ? z >> 1 : z, zb = p.cross ? z & 1 : z;
-  const uint8_t* A = p.a + (size_t)za * p.a_plane + (size_t)tm * BM * p.lda;
-  const uint8_t* B = p.bt + (size_t)zb * p.b_plane + (size_t)tn * BN * p.ldb;
+  const uint8_t* A = p.a + (size_t)z * p.a_plane + (size_t)tm * BM * p.lda;
+  const uint8_t* B = p.bt + (size_t)z * p.b_plane + (size_t)tn * BN * p.ldb;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave / S::WN) * (BM / S::WM), wn0 = (wave % S::WN) * (BN / S::WN);
   const int lr = lane & 31, lk = lane >> 5;
@@ -365,9 +362,8 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
   int tm, tn;
   nt_tile(p.tiles_m, p.tiles_n, tm, tn);
   const int z = blockIdx.y;
-  const int za = p.cross ? z >> 1 : z, zb = p.cross ? z & 1 : z;
-  const uint8_t* A = p.a + (size_t)za * p.a_plane + (size_t)tm * BM * p.lda;
-  const uint8_t* B = p.bt + (size_t)zb * p.b_plane + (size_t)tn * BN * p.ldb;
+  const uint8_t* A = p.a + (size_t)z * p.a_plane + (size_t)tm * BM * p.lda;
+  const uint8_t* B = p.bt + (size_t)z * p.b_plane + (size_t)tn * BN * p.ldb;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm0 = (wave >> 1) * 128, wn0 = (wave & 1) * 128;
