@@ -89,6 +89,9 @@ __device__ __forceinline__ int ssq_shift(Ssq a, int g) {
   return g - (a.E + ex);
 }
 
+__device__ __forceinline__ void block_stats(Ssq acc, int nonfinite, int g, int* pe, double* ps,
+                                            int* anybad, int& shift, int& bad);
+
 // Row scale of one row of A (s_i = g - e_i with ||a_i||_2 <= 2^e_i) and
 // whether it holds a non-finite entry, computed by the whole block; every
 // thread gets the results.
@@ -119,10 +122,17 @@ __device__ __forceinline__ void row_stats(const double* __restrict__ r, int K, i
     acc4[0] = ssq_add(acc4[0], x);
   }
   const Ssq acc = ssq_merge(ssq_merge(acc4[0], acc4[1]), ssq_merge(acc4[2], acc4[3]));
+  block_stats(acc, nonfinite, g, pe, ps, &anybad, shift, bad);
+}
+
+// The block-wide part of row_stats: merges every thread's pair (and
+// non-finite flag) in LDS; anybad must be zero and visible to all threads.
+__device__ __forceinline__ void block_stats(Ssq acc, int nonfinite, int g, int* pe, double* ps,
+                                            int* anybad, int& shift, int& bad) {
   pe[threadIdx.x] = acc.E;
   ps[threadIdx.x] = acc.S;
   __syncthreads();
-  if (nonfinite) atomicOr(&anybad, 1);
+  if (nonfinite) atomicOr(anybad, 1);
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       const Ssq m2 = ssq_merge(Ssq{pe[threadIdx.x], ps[threadIdx.x]},
@@ -133,7 +143,7 @@ __device__ __forceinline__ void row_stats(const double* __restrict__ r, int K, i
     __syncthreads();
   }
   shift = ssq_shift(Ssq{pe[0], ps[0]}, g);
-  bad = anybad;
+  bad = *anybad;
 }
 
 // Column statistics of B (K x N row-major): partial pairs per k-slice
@@ -269,6 +279,82 @@ __global__ __launch_bounds__(256) void oz_scale_residues_a(const double* __restr
     Limbs L[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) L[e] = limbs_of(rint(ldexp(isfinite(x[e]) ? x[e] : 0.0, sh)));
+#pragma unroll
+    for (int l = 0; l < NM; ++l)
+      *reinterpret_cast<unsigned*>(dst_row + k0 + l * plane_bytes) = residue_word(L, c, l);
+  }
+}
+
+// The same for rows of at most 8192 elements (Kp <= 8192), reading A from
+// memory once: thread t holds k = 4t + 1024 g + e (g < 8, e < 4) of its row
+// in registers from the norm to the residues.
+template <int NM>
+__global__ __launch_bounds__(256) void oz_scale_residues_a_reg(const double* __restrict__ a, int M,
+                                                               int K, int Kp, int vec, int g,
+                                                               int* __restrict__ shift,
+                                                               int* __restrict__ bad,
+                                                               uint8_t* __restrict__ planes,
+                                                               size_t plane_bytes) {
+  constexpr int G = 8;
+  const OzConsts& c = g_oz[NM];
+  __shared__ int pe[256];
+  __shared__ double ps[256];
+  __shared__ int anybad;
+  if (threadIdx.x == 0) anybad = 0;
+  const int row = blockIdx.x;
+  const double* src_row = a + (size_t)row * K;
+  double x[G][4];
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi) {
+    const int k0 = 4 * threadIdx.x + 1024 * gi;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) x[gi][e] = 0.0;
+    if (row < M && k0 < Kp) {
+      const double* src = src_row + k0;
+      if (vec && k0 + 4 <= K) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const d2 lo = *reinterpret_cast<const d2*>(src);
+        const d2 hi = *reinterpret_cast<const d2*>(src + 2);
+        x[gi][0] = lo.x;
+        x[gi][1] = lo.y;
+        x[gi][2] = hi.x;
+        x[gi][3] = hi.y;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (k0 + e < K) x[gi][e] = src[e];
+      }
+    }
+  }
+  __syncthreads();  // anybad cleared before any thread sets it
+  int sh = 0, bd = 0;
+  {
+    Ssq acc4[4] = {{INT_MIN, 0.0}, {INT_MIN, 0.0}, {INT_MIN, 0.0}, {INT_MIN, 0.0}};
+    int nonfinite = 0;
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        nonfinite |= !isfinite(x[gi][e]);
+        acc4[e] = ssq_add(acc4[e], x[gi][e]);
+      }
+    const Ssq acc = ssq_merge(ssq_merge(acc4[0], acc4[1]), ssq_merge(acc4[2], acc4[3]));
+    block_stats(acc, nonfinite, g, pe, ps, &anybad, sh, bd);
+  }
+  if (row < M && threadIdx.x == 0) {
+    shift[row] = sh;
+    bad[row] = bd;
+  }
+  if (row >= M) sh = 0;
+  uint8_t* dst_row = planes + (size_t)row * Kp;
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi) {
+    const int k0 = 4 * threadIdx.x + 1024 * gi;
+    if (k0 >= Kp) break;
+    Limbs L[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      L[e] = limbs_of(rint(ldexp(isfinite(x[gi][e]) ? x[gi][e] : 0.0, sh)));
 #pragma unroll
     for (int l = 0; l < NM; ++l)
       *reinterpret_cast<unsigned*>(dst_row + k0 + l * plane_bytes) = residue_word(L, c, l);
@@ -586,8 +672,12 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
   const int vec_a = (k % 2 == 0) && ((uintptr_t)a % 16 == 0);
   with_moduli(nmod, [&](auto nm) {
     constexpr int NM = decltype(nm)::value;
-    hipLaunchKernelGGL((oz_scale_residues_a<NM>), dim3((unsigned)Mp), dim3(256), 0, st, a, (int)m,
-                       (int)k, (int)Kp, vec_a, gamma, sA, badA, ws + o_pa, a_plane);
+    if (Kp <= 8192)
+      hipLaunchKernelGGL((oz_scale_residues_a_reg<NM>), dim3((unsigned)Mp), dim3(256), 0, st, a,
+                         (int)m, (int)k, (int)Kp, vec_a, gamma, sA, badA, ws + o_pa, a_plane);
+    else
+      hipLaunchKernelGGL((oz_scale_residues_a<NM>), dim3((unsigned)Mp), dim3(256), 0, st, a,
+                         (int)m, (int)k, (int)Kp, vec_a, gamma, sA, badA, ws + o_pa, a_plane);
     hipLaunchKernelGGL((oz_residues_bt<NM>), dim3((unsigned)(Np / 32), (unsigned)(Kp / 128)),
                        dim3(256), 0, st, b, (int)k, (int)n, (int)Kp, tB, ws + o_pb, b_plane);
   });
