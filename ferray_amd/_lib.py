@@ -45,6 +45,12 @@ SIGNATURES: dict[str, tuple] = {
     "ferray_gemm_f32_dev": (_i, [_sz, _sz, _sz, _f, _p, _p, _f, _p, _p]),
     "ferray_gemm_c64_dev": (_i, [_sz, _sz, _sz, _d, _d, _p, _p, _d, _d, _p, _p]),
     "ferray_gemm_c32_dev": (_i, [_sz, _sz, _sz, _f, _f, _p, _p, _f, _f, _p, _p]),
+    "ferray_gemm_f64_strided_dev": (_i, [_sz, _sz, _sz, _d, _p, _sz, _p, _sz, _d, _p, _sz, _p]),
+    "ferray_gemm_f32_strided_dev": (_i, [_sz, _sz, _sz, _f, _p, _sz, _p, _sz, _f, _p, _sz, _p]),
+    "ferray_gemm_c64_strided_dev": (_i, [_sz, _sz, _sz, _d, _d, _p, _sz, _p, _sz, _d, _d, _p, _sz,
+                                         _p]),
+    "ferray_gemm_c32_strided_dev": (_i, [_sz, _sz, _sz, _f, _f, _p, _sz, _p, _sz, _f, _f, _p, _sz,
+                                         _p]),
     "ferray_gemm_i8_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
     "ferray_gemm_i8_signed_dev": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),
     "ferray_gemm_i8_signed_dev_noalloc": (_i, [_sz, _sz, _sz, _p, _p, _p, _b, _p]),

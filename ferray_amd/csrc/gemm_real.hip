@@ -111,12 +111,12 @@ void gemm_real_kernel(RealArgs<T> p) {
       const int v = tid + q * R_THREADS, row = v / (BK / V), kc = (v % (BK / V)) * V;
       const int gr = m0 + row;
       if (!EDGE) {
-        ra[q] = *reinterpret_cast<const vec_t*>(p.a + (size_t)gr * K + k0 + kc);
+        ra[q] = *reinterpret_cast<const vec_t*>(p.a + (size_t)gr * p.lda + k0 + kc);
       } else {
 #pragma unroll
         for (int e = 0; e < V; ++e) {
           const int gk = k0 + kc + e;
-          ra[q][e] = (gr < M && gk < K) ? p.a[(size_t)gr * K + gk] : T(0);
+          ra[q][e] = (gr < M && gk < K) ? p.a[(size_t)gr * p.lda + gk] : T(0);
         }
       }
     }
@@ -128,12 +128,12 @@ void gemm_real_kernel(RealArgs<T> p) {
         const int v = tid + q * R_THREADS, kr = v / (RB_N / V), nc = (v % (RB_N / V)) * V;
         const int gk = k0 + kr;
         if (!EDGE) {
-          rb[q] = *reinterpret_cast<const vec_t*>(p.b + (size_t)gk * N + n0 + nc);
+          rb[q] = *reinterpret_cast<const vec_t*>(p.b + (size_t)gk * p.ldb + n0 + nc);
         } else {
 #pragma unroll
           for (int e = 0; e < V; ++e) {
             const int gn = n0 + nc + e;
-            rb[q][e] = (gk < K && gn < N) ? p.b[(size_t)gk * N + gn] : T(0);
+            rb[q][e] = (gk < K && gn < N) ? p.b[(size_t)gk * p.ldb + gn] : T(0);
           }
         }
       }
@@ -144,12 +144,12 @@ void gemm_real_kernel(RealArgs<T> p) {
         const int v = tid + q * R_THREADS, kr = v / (RB_N / V), cc = (v % (RB_N / V)) * (V / 2);
         const int kc = (k0 >> 1) + kr, nc = (n0 >> 1) + cc;
         if (!EDGE) {
-          rb[q] = *reinterpret_cast<const vec_t*>(p.b + ((size_t)kc * Nc + nc) * 2);
+          rb[q] = *reinterpret_cast<const vec_t*>(p.b + (size_t)kc * p.ldb + (size_t)nc * 2);
         } else {
 #pragma unroll
           for (int e = 0; e < V / 2; ++e) {
             const bool ok = kc < Kc && nc + e < Nc;
-            const T* src = p.b + ((size_t)kc * Nc + nc + e) * 2;
+            const T* src = p.b + (size_t)kc * p.ldb + (size_t)(nc + e) * 2;
             rb[q][2 * e] = ok ? src[0] : T(0);
             rb[q][2 * e + 1] = ok ? src[1] : T(0);
           }
@@ -289,7 +289,7 @@ void gemm_real_kernel(RealArgs<T> p) {
         T v = acc[i][j][r];
         if (!CPLX) {
           if (EDGE && (row >= M || col >= N)) continue;
-          T* dst = p.c + (size_t)row * N + col;
+          T* dst = p.c + (size_t)row * p.ldc + col;
           if (!p.alpha_one) v *= p.alpha_re;
           if (!p.beta_zero) v = fma(p.beta_re, *dst, v);
           *dst = v;
@@ -306,7 +306,7 @@ void gemm_real_kernel(RealArgs<T> p) {
             re = vr;
             im = vi;
           }
-          T* dst = p.c + (size_t)row * N + (col & ~1);
+          T* dst = p.c + (size_t)row * p.ldc + (col & ~1);
           if (!p.beta_zero) {
             T cr = dst[0], ci = dst[1];
             T sr = p.beta_re * cr - p.beta_im * ci;
@@ -352,46 +352,81 @@ struct CplxPair {
 };
 
 // Zero-padded copy: dst (rows_p x cols_p, row pitch cols_p) takes src
-// (rows x cols, row pitch cols) in its top-left corner and zeros elsewhere.
+// (rows x cols, row pitch ld) in its top-left corner and zeros elsewhere.
 // Elements are E (the real or complex scalar); x covers columns, y strides
 // over rows.
 template <typename E>
 __global__ void pad_copy_kernel(E* __restrict__ dst, const E* __restrict__ src, size_t rows,
-                                size_t cols, size_t rows_p, size_t cols_p) {
+                                size_t cols, size_t ld, size_t rows_p, size_t cols_p) {
   const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cols_p) return;
   for (size_t r = blockIdx.y; r < rows_p; r += gridDim.y)
-    dst[r * cols_p + c] = (r < rows && c < cols) ? src[r * cols + c] : E{};
+    dst[r * cols_p + c] = (r < rows && c < cols) ? src[r * ld + c] : E{};
 }
 
-// dst (rows x cols, pitch cols) = top-left corner of src (pitch cols_p).
+// dst (rows x cols, pitch ld) = top-left corner of src (pitch cols_p).
 template <typename E>
 __global__ void unpad_copy_kernel(E* __restrict__ dst, const E* __restrict__ src, size_t rows,
-                                  size_t cols, size_t cols_p) {
+                                  size_t cols, size_t ld, size_t cols_p) {
   const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cols) return;
-  for (size_t r = blockIdx.y; r < rows; r += gridDim.y) dst[r * cols + c] = src[r * cols_p + c];
+  for (size_t r = blockIdx.y; r < rows; r += gridDim.y) dst[r * ld + c] = src[r * cols_p + c];
+}
+
+// C = beta * C over an m x n block with row stride ldc (k == 0 with strides).
+template <typename T, bool CPLX>
+__global__ void scale_c_strided_kernel(T* c, size_t m, size_t n, size_t ldc, T beta_re,
+                                       T beta_im) {
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  for (size_t r = blockIdx.y; r < m; r += gridDim.y) {
+    if (!CPLX) {
+      T* e = c + r * ldc + j;
+      *e = beta_re == T(0) ? T(0) : beta_re * *e;
+    } else {
+      T* e = c + 2 * (r * ldc + j);
+      if (beta_re == T(0) && beta_im == T(0)) {
+        e[0] = T(0);
+        e[1] = T(0);
+      } else {
+        const T cr = e[0], ci = e[1];
+        e[0] = beta_re * cr - beta_im * ci;
+        e[1] = beta_re * ci + beta_im * cr;
+      }
+    }
+  }
 }
 
 static dim3 copy_grid(size_t rows, size_t cols) {
   return dim3((unsigned)cdiv(cols, 256), (unsigned)std::min<size_t>(rows, 65535));
 }
 
+// C = alpha A B + beta C with row strides lda >= k, ldb >= n, ldc >= n in
+// elements (complex elements for complex); the tight entry points pass
+// lda = k, ldb = ldc = n.
 template <typename T, bool CPLX>
-static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, const T* b,
-                       T br, T bi, T* c, hipStream_t st) {
+static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, size_t lda,
+                       const T* b, size_t ldb, T br, T bi, T* c, size_t ldc, hipStream_t st) {
   if (m == 0 || n == 0) return FERRAY_OK;
   if (!c || (k && (!a || !b))) return FERRAY_ERR_INVALID;
+  if (lda < k || ldb < n || ldc < n) return FERRAY_ERR_INVALID;
   const size_t mult = CPLX ? 2 : 1;
-  if (m > (1u << 30) || n * mult > (1u << 30) || k * mult > (1u << 30)) return FERRAY_ERR_INVALID;
+  if (m > (1u << 30) || n * mult > (1u << 30) || k * mult > (1u << 30) ||
+      lda * mult > (1u << 30) || ldb * mult > (1u << 30) || ldc * mult > (1u << 30))
+    return FERRAY_ERR_INVALID;
   if (k == 0) {
     const bool beta_one = br == T(1) && bi == T(0);
     if (beta_one) return FERRAY_OK;
-    size_t count = m * n;
-    size_t blocks = cdiv(count, 256);
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL((scale_c_kernel<T, CPLX>), dim3((unsigned)blocks), dim3(256), 0, st, c,
-                       count, br, bi);
+    if (ldc == n) {
+      size_t count = m * n;
+      size_t blocks = cdiv(count, 256);
+      if (blocks > 65536) blocks = 65536;
+      hipLaunchKernelGGL((scale_c_kernel<T, CPLX>), dim3((unsigned)blocks), dim3(256), 0, st, c,
+                         count, br, bi);
+    } else {
+      hipLaunchKernelGGL((scale_c_strided_kernel<T, CPLX>), copy_grid(m, n), dim3(256), 0, st, c,
+                         m, n, ldc, br, bi);
+    }
     FG_CHECK(hipGetLastError());
     return FERRAY_OK;
   }
@@ -402,6 +437,9 @@ static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, con
   p.M = (int)m;
   p.N = (int)(n * mult);
   p.K = (int)(k * mult);
+  p.lda = (int)(lda * mult);
+  p.ldb = (int)(ldb * mult);
+  p.ldc = (int)(ldc * mult);
   p.alpha_re = ar;
   p.alpha_im = ai;
   p.beta_re = br;
@@ -412,7 +450,8 @@ static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, con
   if (tiles > 0x7fffffff) return FERRAY_ERR_INVALID;
   // Small or badly quantised grids: 64 x 64 tiles, bit-identical results.
   if (prefer_small_tiles(tiles)) return launch_real_small<T, CPLX>(p, st);
-  const bool aligned = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0);
+  const bool aligned = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) &&
+                       (lda * mult * sizeof(T)) % 16 == 0 && (ldb * mult * sizeof(T)) % 16 == 0;
   const bool fast = aligned && m % RB_M == 0 && (n * mult) % RB_N == 0 &&
                     (k * mult) % StageK<T>::value == 0;
   if (fast) {
@@ -440,12 +479,12 @@ static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, con
     E* bp = reinterpret_cast<E*>((uint8_t*)ap + cdiv(Mp * Kp * sizeof(E), 256) * 256);
     E* cp = reinterpret_cast<E*>((uint8_t*)bp + cdiv(Kp * Np * sizeof(E), 256) * 256);
     hipLaunchKernelGGL(pad_copy_kernel<E>, copy_grid(Mp, Kp), dim3(256), 0, st, ap,
-                       reinterpret_cast<const E*>(a), m, k, Mp, Kp);
+                       reinterpret_cast<const E*>(a), m, k, lda, Mp, Kp);
     hipLaunchKernelGGL(pad_copy_kernel<E>, copy_grid(Kp, Np), dim3(256), 0, st, bp,
-                       reinterpret_cast<const E*>(b), k, n, Kp, Np);
+                       reinterpret_cast<const E*>(b), k, n, ldb, Kp, Np);
     if (!p.beta_zero)
       hipLaunchKernelGGL(pad_copy_kernel<E>, copy_grid(Mp, Np), dim3(256), 0, st, cp,
-                         reinterpret_cast<const E*>(c), m, n, Mp, Np);
+                         reinterpret_cast<const E*>(c), m, n, ldc, Mp, Np);
     FG_CHECK(hipGetLastError());
     RealArgs<T> q = p;
     q.a = reinterpret_cast<const T*>(ap);
@@ -454,11 +493,14 @@ static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, con
     q.M = (int)Mp;
     q.N = (int)(Np * mult);
     q.K = (int)(Kp * mult);
+    q.lda = q.K;
+    q.ldb = q.N;
+    q.ldc = q.N;
     const size_t ptiles = (Mp / RB_M) * (Np * mult / RB_N);
     hipLaunchKernelGGL((gemm_real_kernel<T, CPLX, false>), dim3((unsigned)ptiles),
                        dim3(R_THREADS), 0, st, q);
     hipLaunchKernelGGL(unpad_copy_kernel<E>, copy_grid(m, n), dim3(256), 0, st,
-                       reinterpret_cast<E*>(c), cp, m, n, Np);
+                       reinterpret_cast<E*>(c), cp, m, n, ldc, Np);
     FG_CHECK(hipGetLastError());
     return FERRAY_OK;
   }
@@ -479,7 +521,7 @@ int ferray_gemm_f64_dev(size_t m, size_t n, size_t k, double alpha, const double
                         const double* b, double beta, double* c, void* stream) {
   int s = ensure_device();
   if (s) return s;
-  return launch_real<double, false>(m, n, k, alpha, 0.0, a, b, beta, 0.0, c,
+  return launch_real<double, false>(m, n, k, alpha, 0.0, a, k, b, n, beta, 0.0, c, n,
                                     resolve_stream(stream));
 }
 
@@ -487,7 +529,7 @@ int ferray_gemm_f32_dev(size_t m, size_t n, size_t k, float alpha, const float* 
                         const float* b, float beta, float* c, void* stream) {
   int s = ensure_device();
   if (s) return s;
-  return launch_real<float, false>(m, n, k, alpha, 0.0f, a, b, beta, 0.0f, c,
+  return launch_real<float, false>(m, n, k, alpha, 0.0f, a, k, b, n, beta, 0.0f, c, n,
                                    resolve_stream(stream));
 }
 
@@ -496,8 +538,8 @@ int ferray_gemm_c64_dev(size_t m, size_t n, size_t k, double alpha_re, double al
                         double* c, void* stream) {
   int s = ensure_device();
   if (s) return s;
-  return launch_real<double, true>(m, n, k, alpha_re, alpha_im, a, b, beta_re, beta_im, c,
-                                   resolve_stream(stream));
+  return launch_real<double, true>(m, n, k, alpha_re, alpha_im, a, k, b, n, beta_re, beta_im, c,
+                                   n, resolve_stream(stream));
 }
 
 int ferray_gemm_c32_dev(size_t m, size_t n, size_t k, float alpha_re, float alpha_im,
@@ -505,8 +547,46 @@ int ferray_gemm_c32_dev(size_t m, size_t n, size_t k, float alpha_re, float alph
                         void* stream) {
   int s = ensure_device();
   if (s) return s;
-  return launch_real<float, true>(m, n, k, alpha_re, alpha_im, a, b, beta_re, beta_im, c,
-                                  resolve_stream(stream));
+  return launch_real<float, true>(m, n, k, alpha_re, alpha_im, a, k, b, n, beta_re, beta_im, c,
+                                  n, resolve_stream(stream));
+}
+
+int ferray_gemm_f64_strided_dev(size_t m, size_t n, size_t k, double alpha, const double* a,
+                                size_t lda, const double* b, size_t ldb, double beta, double* c,
+                                size_t ldc, void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  return launch_real<double, false>(m, n, k, alpha, 0.0, a, lda, b, ldb, beta, 0.0, c, ldc,
+                                    resolve_stream(stream));
+}
+
+int ferray_gemm_f32_strided_dev(size_t m, size_t n, size_t k, float alpha, const float* a,
+                                size_t lda, const float* b, size_t ldb, float beta, float* c,
+                                size_t ldc, void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  return launch_real<float, false>(m, n, k, alpha, 0.0f, a, lda, b, ldb, beta, 0.0f, c, ldc,
+                                   resolve_stream(stream));
+}
+
+int ferray_gemm_c64_strided_dev(size_t m, size_t n, size_t k, double alpha_re, double alpha_im,
+                                const double* a, size_t lda, const double* b, size_t ldb,
+                                double beta_re, double beta_im, double* c, size_t ldc,
+                                void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  return launch_real<double, true>(m, n, k, alpha_re, alpha_im, a, lda, b, ldb, beta_re, beta_im,
+                                   c, ldc, resolve_stream(stream));
+}
+
+int ferray_gemm_c32_strided_dev(size_t m, size_t n, size_t k, float alpha_re, float alpha_im,
+                                const float* a, size_t lda, const float* b, size_t ldb,
+                                float beta_re, float beta_im, float* c, size_t ldc,
+                                void* stream) {
+  int s = ensure_device();
+  if (s) return s;
+  return launch_real<float, true>(m, n, k, alpha_re, alpha_im, a, lda, b, ldb, beta_re, beta_im,
+                                  c, ldc, resolve_stream(stream));
 }
 
 }  // extern "C"

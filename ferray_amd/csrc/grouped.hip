@@ -192,6 +192,9 @@ __device__ __noinline__ void real_dispatch(const GProblem& P, int m0, int n0, ui
   r.M = P.m;
   r.N = CPLX ? 2 * P.n : P.n;
   r.K = CPLX ? 2 * P.k : P.k;
+  r.lda = r.K;
+  r.ldb = r.N;
+  r.ldc = r.N;
   r.alpha_re = (T)P.ar;
   r.alpha_im = (T)P.ai;
   r.beta_re = (T)P.br;

@@ -51,6 +51,7 @@ struct RealArgs {
   const T* b;
   T* c;
   int M, N, K;  // real dimensions (complex: N = 2n, K = 2k)
+  int lda, ldb, ldc;  // row strides in scalars of T (complex: twice the element stride)
   T alpha_re, alpha_im, beta_re, beta_im;
   int alpha_one;  // alpha == 1 (+0i): skip the scaling
   int beta_zero;  // beta == 0: C is written without being read
@@ -109,12 +110,12 @@ __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, 
       const int row = v / (BK / V), kc = (v % (BK / V)) * V;
       const int gr = m0 + row;
       if (!EDGE) {
-        ra[q] = *reinterpret_cast<const vec_t*>(p.a + (size_t)gr * K + k0 + kc);
+        ra[q] = *reinterpret_cast<const vec_t*>(p.a + (size_t)gr * p.lda + k0 + kc);
       } else {
 #pragma unroll
         for (int e = 0; e < V; ++e) {
           const int gk = k0 + kc + e;
-          ra[q][e] = (gr < M && gk < K) ? p.a[(size_t)gr * K + gk] : T(0);
+          ra[q][e] = (gr < M && gk < K) ? p.a[(size_t)gr * p.lda + gk] : T(0);
         }
       }
     }
@@ -126,12 +127,12 @@ __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, 
         const int kr = v / (BN / V), nc = (v % (BN / V)) * V;
         const int gk = k0 + kr;
         if (!EDGE) {
-          rb[q] = *reinterpret_cast<const vec_t*>(p.b + (size_t)gk * N + n0 + nc);
+          rb[q] = *reinterpret_cast<const vec_t*>(p.b + (size_t)gk * p.ldb + n0 + nc);
         } else {
 #pragma unroll
           for (int e = 0; e < V; ++e) {
             const int gn = n0 + nc + e;
-            rb[q][e] = (gk < K && gn < N) ? p.b[(size_t)gk * N + gn] : T(0);
+            rb[q][e] = (gk < K && gn < N) ? p.b[(size_t)gk * p.ldb + gn] : T(0);
           }
         }
       } else {
@@ -139,12 +140,12 @@ __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, 
         const int kr = v / (BN / V), cc = (v % (BN / V)) * (V / 2);
         const int kc = (k0 >> 1) + kr, nc = (n0 >> 1) + cc;
         if (!EDGE) {
-          rb[q] = *reinterpret_cast<const vec_t*>(p.b + ((size_t)kc * Nc + nc) * 2);
+          rb[q] = *reinterpret_cast<const vec_t*>(p.b + (size_t)kc * p.ldb + (size_t)nc * 2);
         } else {
 #pragma unroll
           for (int e = 0; e < V / 2; ++e) {
             const bool ok = kc < Kc && nc + e < Nc;
-            const T* src = p.b + ((size_t)kc * Nc + nc + e) * 2;
+            const T* src = p.b + (size_t)kc * p.ldb + (size_t)(nc + e) * 2;
             rb[q][2 * e] = ok ? src[0] : T(0);
             rb[q][2 * e + 1] = ok ? src[1] : T(0);
           }
@@ -261,7 +262,7 @@ __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, 
         T v = acc[i][j][r];
         if (!CPLX) {
           if (EDGE && (row >= M || col >= N)) continue;
-          T* dst = p.c + (size_t)row * N + col;
+          T* dst = p.c + (size_t)row * p.ldc + col;
           if (!p.alpha_one) v *= p.alpha_re;
           if (!p.beta_zero) v = fma(p.beta_re, *dst, v);
           *dst = v;
@@ -277,7 +278,7 @@ __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, 
             re = vr;
             im = vi;
           }
-          T* dst = p.c + (size_t)row * N + (col & ~1);
+          T* dst = p.c + (size_t)row * p.ldc + (col & ~1);
           T outv;
           if (!p.beta_zero) {
             const T cr = dst[0], ci = dst[1];
@@ -297,8 +298,8 @@ __device__ __forceinline__ void real_tile(const RealArgs<T>& p, int m0, int n0, 
 // Whether the tile at (m0, n0) can take the unguarded path.
 template <typename T, int BM, int BN, int BK>
 __device__ __forceinline__ bool real_tile_interior(const RealArgs<T>& p, int m0, int n0) {
-  return m0 + BM <= p.M && n0 + BN <= p.N && p.K % BK == 0 && (p.K * sizeof(T)) % 16 == 0 &&
-         (p.N * sizeof(T)) % 16 == 0 && ((uintptr_t)p.a % 16) == 0 && ((uintptr_t)p.b % 16) == 0;
+  return m0 + BM <= p.M && n0 + BN <= p.N && p.K % BK == 0 && (p.lda * sizeof(T)) % 16 == 0 &&
+         (p.ldb * sizeof(T)) % 16 == 0 && ((uintptr_t)p.a % 16) == 0 && ((uintptr_t)p.b % 16) == 0;
 }
 
 // Host launcher of the 64 x 64 tile kernel for products too small to fill

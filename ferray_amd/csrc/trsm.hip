@@ -5,8 +5,8 @@
 // Same block recursion as the reference (trsm.rs:165-200, :264-301): split
 // the rows at m1 = min(round_up(m/2, NB), m - NB), solve one half, update the
 // other with a GEMM, solve it.  Here the trailing update runs on the MFMA
-// DGEMM/SGEMM (C = -1 * A21 X1 + 1 * B2; strided blocks are first copied to
-// tight workspace with 2-D copies), and the leaf (m <= NB = 64 rows) is a
+// DGEMM/SGEMM (C = -1 * A21 X1 + 1 * B2, on the strided blocks where they
+// lie, through the row-strided GEMM), and the leaf (m <= NB = 64 rows) is a
 // kernel in which each thread solves one column of B by substitution with the
 // triangular block in LDS.  The leaf applies the reference's scalar
 // operation order (row i: subtract a_ip x_p for p in order, then multiply by
@@ -131,18 +131,21 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int m, int n, const T* _
     if (i < m) b[(size_t)i * ldb + j] = x[i];
 }
 
+// Strided GEMM on the device (the drivers' row-strided form).
 template <typename T>
-int gemm_dev(size_t m, size_t n, size_t k, T alpha, const T* a, const T* b, T beta, T* c,
-             hipStream_t st);
+int gemm_strided(size_t m, size_t n, size_t k, T alpha, const T* a, size_t lda, const T* b,
+                 size_t ldb, T beta, T* c, size_t ldc, hipStream_t st);
 template <>
-int gemm_dev<double>(size_t m, size_t n, size_t k, double alpha, const double* a, const double* b,
-                     double beta, double* c, hipStream_t st) {
-  return ferray_gemm_f64_dev(m, n, k, alpha, a, b, beta, c, st);
+int gemm_strided<double>(size_t m, size_t n, size_t k, double alpha, const double* a, size_t lda,
+                         const double* b, size_t ldb, double beta, double* c, size_t ldc,
+                         hipStream_t st) {
+  return ferray_gemm_f64_strided_dev(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, st);
 }
 template <>
-int gemm_dev<float>(size_t m, size_t n, size_t k, float alpha, const float* a, const float* b,
-                    float beta, float* c, hipStream_t st) {
-  return ferray_gemm_f32_dev(m, n, k, alpha, a, b, beta, c, st);
+int gemm_strided<float>(size_t m, size_t n, size_t k, float alpha, const float* a, size_t lda,
+                        const float* b, size_t ldb, float beta, float* c, size_t ldc,
+                        hipStream_t st) {
+  return ferray_gemm_f32_strided_dev(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, st);
 }
 
 template <typename T>
@@ -150,9 +153,6 @@ struct TrsmCtx {
   int n;
   size_t lda, ldb;
   bool upper, unit;
-  T* ws_a;  // tight copy of the off-diagonal block
-  T* ws_x;  // tight copies of the solved and the updated rows (ldb != n)
-  T* ws_b;
   hipStream_t st;
 };
 
@@ -161,33 +161,12 @@ static size_t trsm_split(size_t m) {
   return std::min(half, m - TRSM_NB);
 }
 
-// dst (rows x cols, tight) <- src (row stride ld), or the reverse.
-template <typename T>
-static int copy2d(T* dst, size_t dld, const T* src, size_t sld, size_t rows, size_t cols,
-                  hipStream_t st) {
-  if (!rows || !cols) return FERRAY_OK;
-  FG_CHECK(hipMemcpy2DAsync(dst, dld * sizeof(T), src, sld * sizeof(T), cols * sizeof(T), rows,
-                            hipMemcpyDeviceToDevice, st));
-  return FERRAY_OK;
-}
-
-// B_t -= A_ts X_s: A_ts (rt x rs) at a (stride lda), X_s (rs x n) and B_t
-// (rt x n) rows of B (stride ldb).
+// B_t -= A_ts X_s in place (trsm.rs:205-214): A_ts (rt x rs) at a (stride
+// lda), X_s (rs x n) and B_t (rt x n) rows of B (stride ldb), read and
+// written where they lie by the strided GEMM.
 template <typename T>
 static int trsm_update(const TrsmCtx<T>& c, size_t rt, size_t rs, const T* a, const T* xs, T* bt) {
-  int s;
-  if ((s = copy2d(c.ws_a, rs, a, c.lda, rt, rs, c.st))) return s;
-  const T* x = xs;
-  T* bb = bt;
-  if (c.ldb != (size_t)c.n) {
-    if ((s = copy2d(c.ws_x, (size_t)c.n, xs, c.ldb, rs, (size_t)c.n, c.st))) return s;
-    if ((s = copy2d(c.ws_b, (size_t)c.n, bt, c.ldb, rt, (size_t)c.n, c.st))) return s;
-    x = c.ws_x;
-    bb = c.ws_b;
-  }
-  if ((s = gemm_dev<T>(rt, c.n, rs, T(-1), c.ws_a, x, T(1), bb, c.st))) return s;
-  if (c.ldb != (size_t)c.n) return copy2d(bt, c.ldb, c.ws_b, (size_t)c.n, rt, (size_t)c.n, c.st);
-  return FERRAY_OK;
+  return gemm_strided<T>(rt, (size_t)c.n, rs, T(-1), a, c.lda, xs, c.ldb, T(1), bt, c.ldb, c.st);
 }
 
 template <typename T>
@@ -232,20 +211,7 @@ static int trsm_dev(bool upper, size_t m, size_t n, T alpha, const T* a, size_t 
                        (int)n, alpha, b, ldb);
     FG_CHECK(hipGetLastError());
   }
-  TrsmCtx<T> c{(int)n, lda, ldb, upper, unit_diag, nullptr, nullptr, nullptr, st};
-  if (m > (size_t)TRSM_NB) {
-    // The top split has the largest blocks.
-    const size_t m1 = trsm_split(m), m2 = m - m1;
-    const size_t blk = m1 * m2;
-    const size_t pack = ldb != n ? m * n : 0;
-    const size_t a_bytes = (blk * sizeof(T) + 255) / 256 * 256;
-    void* ws = nullptr;
-    int s = workspace(st, a_bytes + 2 * pack * sizeof(T) + 256, &ws);
-    if (s) return s;
-    c.ws_a = reinterpret_cast<T*>(ws);
-    c.ws_x = reinterpret_cast<T*>(reinterpret_cast<uint8_t*>(ws) + a_bytes);
-    c.ws_b = c.ws_x + pack;
-  }
+  TrsmCtx<T> c{(int)n, lda, ldb, upper, unit_diag, st};
   return trsm_rec(c, m, a, b);
 }
 

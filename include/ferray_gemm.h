@@ -156,6 +156,34 @@ int ferray_gemm_e5m2_f32_dev(size_t m, size_t n, size_t k, const uint8_t *a,
                              const uint8_t *b, float *c, bool accumulate,
                              void *stream);
 
+/* Row-strided forms of the four floating-point device entries, the
+ * drivers' own signatures: C = alpha*A@B + beta*C with A m x k (row stride
+ * lda >= k), B k x n (ldb >= n), C m x n (ldc >= n), strides in elements
+ * (complex elements for c64/c32); elements of C outside its m x n block are
+ * not touched.  Replace outer::gemm_avx2_f64 (outer.rs:198-210),
+ * gemm_avx2_f32 (outer.rs:499-511), gemm_avx2_c64 (outer.rs:1231-1245) and
+ * gemm_avx2_c32 (outer.rs:1484-1498), which the TRSM trailing update calls
+ * on strided blocks (trsm.rs:205-214).  Results equal the tight entries'
+ * bit for bit. */
+int ferray_gemm_f64_strided_dev(size_t m, size_t n, size_t k, double alpha,
+                                const double *a, size_t lda, const double *b,
+                                size_t ldb, double beta, double *c, size_t ldc,
+                                void *stream);
+int ferray_gemm_f32_strided_dev(size_t m, size_t n, size_t k, float alpha,
+                                const float *a, size_t lda, const float *b,
+                                size_t ldb, float beta, float *c, size_t ldc,
+                                void *stream);
+int ferray_gemm_c64_strided_dev(size_t m, size_t n, size_t k, double alpha_re,
+                                double alpha_im, const double *a, size_t lda,
+                                const double *b, size_t ldb, double beta_re,
+                                double beta_im, double *c, size_t ldc,
+                                void *stream);
+int ferray_gemm_c32_strided_dev(size_t m, size_t n, size_t k, float alpha_re,
+                                float alpha_im, const float *a, size_t lda,
+                                const float *b, size_t ldb, float beta_re,
+                                float beta_im, float *c, size_t ldc,
+                                void *stream);
+
 /* ---------------- triangular solve (GEMM consumer) ---------------- */
 
 /* Solves L X = alpha B in place: L lower triangular m x m (row stride
