@@ -11,7 +11,9 @@ Besides the headline the line carries:
                   rocprofv3 counter summary for it is present under profiles/;
   cpu_baseline  - the oracle's restatement of ferray's CPU GEMM (Goto 5-loop,
                   AVX2 4x8 kernel, min(nproc, 8) workers) timed on this host
-                  on a bounded 4096^3 f64 sample;
+                  at 8192^3 f64 (median of 3 after one warm-up), plus the
+                  512^3 config timed the reference's way (median of 30 after
+                  5 warm-ups) beside ferray's published figure;
   error         - normwise error of the 8192 result against a compensated
                   (double-double) dot product on sampled rows and columns;
   split_precision - the Ozaki-scheme DGEMM on INT8 MFMA at the same size, per
@@ -256,24 +258,98 @@ def consumers(fa, steps: int, warmup: int):
     return out
 
 
-def cpu_baseline(n: int = 4096, reps: int = 3):
+def _median_time(fn, reps: int, warm: int) -> float:
+    for _ in range(warm):
+        fn()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times))
+
+
+def cpu_baseline(n: int = 8192, reps: int = 3):
+    """The oracle's restatement of ferray's CPU DGEMM on this host at the
+    headline size (median of `reps` after one warm-up), plus config 1
+    (512^3, the reference's LCG inputs) timed the reference's way: median of
+    30 after 5 warm-ups, beside ferray's published 1768 us on 8 threads
+    (outer.rs:136-141)."""
     import oracle
     a = oracle.uniform_f64(n * n, 1)
     b = oracle.uniform_f64(n * n, 2)
     c = np.zeros(n * n)
-    workers = oracle.gemm_f64_goto(n, n, n, 1.0, a, b, 0.0, c)   # warm-up
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        oracle.gemm_f64_goto(n, n, n, 1.0, a, b, 0.0, c)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return {"value": 2.0 * n ** 3 / t / 1e12, "unit": "TFLOP/s", "cores": int(workers),
+    workers = oracle.gemm_f64_goto(n, n, n, 1.0, a, b, 0.0, c)
+    t = _median_time(lambda: oracle.gemm_f64_goto(n, n, n, 1.0, a, b, 0.0, c), reps, 0)
+    del a, b, c
+    m = 512
+    a1, b1 = oracle.lcg_matrix(m, 1), oracle.lcg_matrix(m, 2)
+    c1, ref = np.zeros(m * m), np.zeros(m * m)
+    t1 = _median_time(lambda: oracle.gemm_f64_goto(m, m, m, 1.0, a1, b1, 0.0, c1), 30, 5)
+    oracle.gemm_f64(m, m, m, 1.0, a1, b1, 0.0, ref)
+    return {"value": round(2.0 * n ** 3 / t / 1e12, 4), "unit": "TFLOP/s", "cores": int(workers),
             "kind": "port",
             "sample": (f"ferray CPU DGEMM restated in C (Goto 5-loop MC/KC/NC=64/256/512, AVX2 4x8, "
                        f"pick_grid, min(nproc,8) workers); f64 {n}^3 uniform[-1,1), median of "
                        f"{reps} after 1 warm-up, {t * 1e3:.0f} ms per call; avx2="
-                       f"{oracle.cpu_has_avx2_fma()}")}
+                       f"{oracle.cpu_has_avx2_fma()}"),
+            "config1_512": {
+                "us": round(t1 * 1e6, 1), "gflops": round(2.0 * m ** 3 / t1 / 1e9, 1),
+                "method": "LCG seeds 1, 2 (examples/bench_matmul.rs:21-34); median of 30 after 5 "
+                          "warm-ups",
+                "bit_identical_to_block_formula": bool(np.array_equal(c1, ref)),
+                "ferray_published_us_8_threads": 1768.0}}
+
+
+def offload_crossover(fa):
+    """Host-pointer GPU calls (upload, product, download, synchronise) against
+    the CPU restatement of ferray's GEMM on this host, square products of
+    size n: the smallest n from which the GPU call is faster at that and
+    every larger size, per dtype.  The CPU side follows ferray's dispatch:
+    one thread below max-dim 256 (faer Par::Seq, products/mod.rs:499-576),
+    min(nproc, 8) workers from 256 (the hand-tuned pool, outer.rs:142-155).
+    Complex products are timed on the CPU as the real Goto GEMM of the same
+    flop count (m x 2n x 2k): the restatement's complex kernel is the plain
+    block formula, not ferray's AVX2 one."""
+    import oracle
+    sizes = [8, 16, 32, 48, 64, 96, 128, 192, 256, 384, 512]
+    out = {}
+    for dt in ("f64", "f32", "c64", "c32"):
+        real = np.float64 if dt in ("f64", "c64") else np.float32
+        cplx = dt[0] == "c"
+        rows, cross = [], None
+        for n in sizes:
+            r = np.random.default_rng(n)
+            if cplx:
+                a = r.standard_normal(2 * n * n).astype(real)
+                b = r.standard_normal(2 * n * n).astype(real)
+                c = np.zeros(2 * n * n, real)
+                gpu = (fa.gemm_c64 if dt == "c64" else fa.gemm_c32)
+                g = lambda: gpu(n, n, n, 1.0, 0.0, a, b, 0.0, 0.0, c)  # noqa: E731
+                ca = r.standard_normal(4 * n * n).astype(real)
+                cc = np.zeros(4 * n * n, real)
+                cpu_f = oracle.gemm_f64_goto if real == np.float64 else oracle.gemm_f32_goto
+                h = lambda: cpu_f(n, 2 * n, 2 * n, real(1.0), ca, ca, real(0.0), cc,  # noqa: E731
+                                  1 if n < 256 else 0)
+            else:
+                a = r.standard_normal(n * n).astype(real)
+                b = r.standard_normal(n * n).astype(real)
+                c = np.zeros(n * n, real)
+                gpu = fa.gemm_f64 if dt == "f64" else fa.gemm_f32
+                g = lambda: gpu(n, n, n, real(1.0), a, b, real(0.0), c)  # noqa: E731
+                cpu_f = oracle.gemm_f64_goto if dt == "f64" else oracle.gemm_f32_goto
+                h = lambda: cpu_f(n, n, n, real(1.0), a, b, real(0.0), c,  # noqa: E731
+                                  1 if n < 256 else 0)
+            tg = _median_time(g, 15, 3)
+            tc = _median_time(h, 15, 3)
+            rows.append({"n": n, "gpu_us": round(tg * 1e6, 1), "cpu_us": round(tc * 1e6, 1)})
+        for i in range(len(rows)):
+            if all(r["gpu_us"] < r["cpu_us"] for r in rows[i:]):
+                cross = rows[i]["n"]
+                break
+        out[dt] = {"crossover_n": cross, "points": rows}
+        log(f"offload crossover {dt}: n = {cross}")
+    return out
 
 
 def run_sweep(fa, steps: int, warmup: int):
@@ -393,6 +469,9 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--no-consumers", action="store_true")
+    ap.add_argument("--crossover", action="store_true",
+                    help="also time host-pointer products against the CPU restatement to find "
+                         "the size from which offloading pays, per dtype")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 1 GiB device-to-device copy, the reference point "
                          "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
@@ -468,6 +547,28 @@ def main() -> int:
             "note": "ferray_gemm_f64 on pageable host buffers: H2D of A, B, DGEMM, D2H of C"}
         log(f"drop-in DGEMM with host buffers: {w * 1e3:.1f} ms ({flops / w / 1e12:.2f} TFLOP/s)")
         del a_h, b_h, c_h
+    if not args.no_host:
+        # Config 1: the reference's own 512^3 LCG case, on the device.
+        import oracle
+        m1 = 512
+        a1, b1 = oracle.lcg_matrix(m1, 1), oracle.lcg_matrix(m1, 2)
+        c1 = np.zeros(m1 * m1)
+        d1a, d1b, d1c = (fa.DeviceBuffer(m1 * m1 * 8) for _ in range(3))
+        d1a.upload(a1)
+        d1b.upload(b1)
+        dev_ms = time_steps(fa, lambda: L.ferray_gemm_f64_dev(m1, m1, m1, 1.0, d1a.ptr, d1b.ptr,
+                                                              0.0, d1c.ptr, None), 30, 5)
+        host_s = _median_time(lambda: fa.gemm_f64(m1, m1, m1, 1.0, a1, b1, 0.0, c1), 30, 5)
+        for buf in (d1a, d1b, d1c):
+            buf.free()
+        result["config1_512"] = {
+            "device_resident_us": round(dev_ms * 1e3, 1),
+            "host_pointers_us": round(host_s * 1e6, 1),
+            "note": "LCG inputs of examples/bench_matmul.rs; device time by HIP events over 30 "
+                    "calls after 5 warm-ups; host-pointer call median of 30 after 5 (PCIe "
+                    "included)"}
+        log(f"config-1 512^3: {dev_ms * 1e3:.1f} us resident, {host_s * 1e6:.1f} us with host "
+            f"buffers")
     if not args.no_error:
         log("sampled error against compensated dot products")
         result["error"] = dgemm_error(fa, dA, dB, dC, n)
@@ -491,6 +592,8 @@ def main() -> int:
         b = result["serving"]["best_at_p99_target"]["grouped"]
         if b:
             result["serving_p99_ms"] = b["p99_ms"]
+    if args.crossover:
+        result["offload_crossover"] = offload_crossover(fa)
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle restatement of ferray's Goto DGEMM)")
         result["cpu_baseline"] = cpu_baseline()

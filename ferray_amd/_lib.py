@@ -39,6 +39,7 @@ SIGNATURES: dict[str, tuple] = {
     "ferray_gemm_e4m3_f32": (_b, [_sz, _sz, _sz, _p, _p, _p, _b]),
     "ferray_gemm_e5m2_f32": (_b, [_sz, _sz, _sz, _p, _p, _p, _b]),
     "ferray_gpu_available": (_b, []),
+    "ferray_gemm_offload_min_dim": (_sz, [_i]),
     "ferray_gemm_last_status": (_i, []),
     "ferray_status_string": (C.c_char_p, [_i]),
     "ferray_gemm_f64_dev": (_i, [_sz, _sz, _sz, _d, _p, _p, _d, _p, _p]),

@@ -204,6 +204,16 @@ const char* ferray_status_string(int status) {
 
 bool ferray_gpu_available(void) { return ensure_device() == FERRAY_OK; }
 
+size_t ferray_gemm_offload_min_dim(int dtype) {
+  static const size_t kMeasured[] = {128, 96, 96, 64, 96};  // F64, F32, C64, C32, I8
+  if (dtype < FERRAY_DTYPE_F64 || dtype > FERRAY_DTYPE_I8) return 0;
+  if (const char* env = std::getenv("FERRAY_OFFLOAD_MIN_DIM")) {
+    const long long v = std::atoll(env);
+    if (v > 0) return (size_t)v;
+  }
+  return kMeasured[dtype];
+}
+
 int ferray_dev_malloc(size_t bytes, void** out) {
   int s = ensure_device();
   if (s) return s;

@@ -19,7 +19,7 @@ import numpy as np
 from ._lib import check, lib
 
 __all__ = [
-    "gpu_supports", "gemm_f64", "gemm_f32", "gemm_c64", "gemm_c32", "gemm_i8",
+    "gpu_supports", "offload_min_dim", "gemm_f64", "gemm_f32", "gemm_c64", "gemm_c32", "gemm_i8",
     "gemm_i8_signed", "gemm_i16", "quantized_matmul", "gemm_bf16_f32",
     "gemm_f16_f32", "gemm_e4m3_f32", "gemm_e5m2_f32", "dgemm_ozaki", "last_status", "DeviceBuffer", "fill", "sync",
     "sgemm_bf16x3", "trsm_left_lower_f64", "trsm_left_upper_f64",
@@ -52,6 +52,14 @@ def _need(arr: np.ndarray, dtype, count: int, name: str, writable: bool = False)
 def gpu_supports() -> bool:
     """Device analogue of gemm::cpu_supports_avx2_fma (mod.rs:118-188)."""
     return bool(lib().ferray_gpu_available())
+
+
+def offload_min_dim(dtype: int) -> int:
+    """Smallest max(m, n, k) from which a host-pointer call pays off against
+    ferray's CPU GEMM (measured on MI355X; FERRAY_OFFLOAD_MIN_DIM overrides):
+    the counterpart of the reference's size dispatch (products/mod.rs:445,
+    :499; complex.rs:160) for callers that keep their CPU kernels below it."""
+    return int(lib().ferray_gemm_offload_min_dim(int(dtype)))
 
 
 def last_status() -> int:

@@ -87,6 +87,19 @@ bool ferray_gemm_e4m3_f32(size_t m, size_t n, size_t k, const uint8_t *a,
 bool ferray_gemm_e5m2_f32(size_t m, size_t n, size_t k, const uint8_t *a,
                           const uint8_t *b, float *c, bool accumulate);
 
+/* Size-threshold policy for host callers: the smallest max(m, n, k) from
+ * which a host-pointer call (upload, product, download) beats ferray's CPU
+ * GEMM, measured on MI355X against the CPU restatement under ferray's own
+ * dispatch (one thread below max-dim 256, the pool from 256): 128 for
+ * FERRAY_DTYPE_F64, 96 for F32 and C64, 64 for C32; I8 (not measured
+ * separately) uses the F32 figure.  The environment variable
+ * FERRAY_OFFLOAD_MIN_DIM, when set to a positive integer, overrides it for
+ * every dtype.  The analogue of the reference's size dispatch
+ * (products/mod.rs:445, :478-499 naive / faer / hand-tuned; complex.rs:160):
+ * callers keep ferray's CPU kernels below it.  Returns 0 for an unknown
+ * dtype. */
+size_t ferray_gemm_offload_min_dim(int dtype);
+
 /* Device probe: the analogue of gemm::cpu_supports_avx2_fma
  * (mod.rs:118-188) — true iff a gfx950 device is usable. */
 bool ferray_gpu_available(void);
