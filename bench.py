@@ -416,6 +416,36 @@ def run_sweep(fa, steps: int, warmup: int):
 SERVE_P99_TARGET_MS = 2.0
 
 
+def serving_check(fa, sv, pools, count: int = 500):
+    """Served results against their single-problem entry points, bit for bit,
+    on `count` requests of the serving trace kept by a keep_results server."""
+    L = fa.lib()
+    reqs, arrival, _ = sv.make_trace(pools, count, rate_per_s=1e5, seed=11)
+    cfg = sv.config(hbm_bytes=4 << 30, max_resident=4096, max_batch=256, lanes=4,
+                    keep_results=True)
+    with sv.Server(cfg) as srv:
+        tickets = [srv.submit(q) for q in reqs]
+        got = [srv.result(t, q.dtype, q.m, q.n) for t, q in zip(tickets, reqs)]
+    out = fa.DeviceBuffer(max(q.m * q.n * 16 for q in reqs))
+    same = 0
+    for q, c in zip(reqs, got):
+        if q.dtype == sv.DTYPE_F64:
+            st = L.ferray_gemm_f64_dev(q.m, q.n, q.k, q.alpha_re, q.a, q.b, 0.0, out.ptr, None)
+        elif q.dtype == sv.DTYPE_F32:
+            st = L.ferray_gemm_f32_dev(q.m, q.n, q.k, q.alpha_re, q.a, q.b, 0.0, out.ptr, None)
+        elif q.dtype == sv.DTYPE_C64:
+            st = L.ferray_gemm_c64_dev(q.m, q.n, q.k, q.alpha_re, q.alpha_im, q.a, q.b, 0.0,
+                                       0.0, out.ptr, None)
+        else:
+            st = L.ferray_gemm_i8_signed_dev(q.m, q.n, q.k, q.a, q.b, out.ptr, False, None)
+        fa.sync()
+        if st == 0 and out.download(sv.c_dtype(q.dtype), q.m * q.n).tobytes() == c.tobytes():
+            same += 1
+    out.free()
+    log(f"serving result check: {same} / {count} bit-identical to the single-problem entries")
+    return {"requests": count, "bit_identical_to_single_entry": same, "ok": same == count}
+
+
 def serving(fa, count: int = 20000):
     """The serving workload: `count` requests, Poisson arrivals, M, N, K
     log-uniform in [16, 2048], 50% f64 / 30% f32 / 10% c64 / 10% i8->i32,
@@ -423,10 +453,16 @@ def serving(fa, count: int = 20000):
     arrival rate is replayed through a fresh server, grouped (continuous
     batching onto 4 launch lanes) and one launch per request."""
     from ferray_amd import serve as sv
-    pools = sv.OperandPools(fa, elems=1 << 24, seed=7)
+    elems = 1 << 29
+    pools = sv.OperandPools(fa, elems=elems, seed=7)
     out = {"requests": count, "dims": "log-uniform [16, 2048]",
            "mix": "f64 0.5, f32 0.3, c64 0.1, i8 0.1", "p99_target_ms": SERVE_P99_TARGET_MS,
-           "lanes": 4, "max_batch": 256}
+           "lanes": 4, "max_batch": 256,
+           "operand_pools_gib": round(elems * (8 + 4 + 1) / 2 ** 30, 2),
+           "operands": "each request reads A and B at random 256-byte-aligned offsets of "
+                       "device pools (f64 4 GiB, f32 2 GiB, i8 0.5 GiB) far larger than the "
+                       "256 MB last-level cache, so operands stream from HBM"}
+    out["result_check"] = serving_check(fa, sv, pools)
     rates = {"grouped": [2e4, 5e4, 7.5e4, 1e5, 1.25e5, 1.5e5, 2e5],
              "per_request": [1e4, 2e4, 3e4, 3.5e4, 4e4, 5e4, 6e4]}
     best = {}

@@ -255,14 +255,23 @@ void grouped_kernel(const uint8_t* __restrict__ desc) {
 // Host side
 // ---------------------------------------------------------------------
 
-// Tile-shape table: cfg (0 = 64x64, 1 = 128x128) per dtype and size bucket
-// b = clamp(ceil(log2(max(m, n_real))), 4, 11) - 4.  Defaults: 128x128 from
-// 256 up; set by ferray_set_tile_table (the autotuner's output).
-constexpr int G_DTYPES = 5, G_BUCKETS = 8;
+// Tile-shape table: cfg (0 = 64x64, 1 = 128x128) per dtype, size bucket
+// b = clamp(ceil(log2(max(m, n_real))), 4, 11) - 4, small-side class
+// s = [min(m, n_real) < 64, < 128, >= 128] and depth class
+// q = [k_real < 128, < 512, >= 512] (k_real = 2k for complex).  Defaults:
+// 128x128 from max side 256 up; set by ferray_set_tile_table (the autotuner's
+// output).
+constexpr int G_DTYPES = 5, G_BUCKETS = 8, G_SMALL = 3, G_KCLS = 3;
+constexpr int G_CELLS = G_BUCKETS * G_SMALL * G_KCLS;
 static std::mutex g_table_mu;
-static int g_table[G_DTYPES][G_BUCKETS] = {
-    {0, 0, 0, 0, 1, 1, 1, 1}, {0, 0, 0, 0, 1, 1, 1, 1}, {0, 0, 0, 0, 1, 1, 1, 1},
-    {0, 0, 0, 0, 1, 1, 1, 1}, {0, 0, 0, 0, 1, 1, 1, 1}};
+static int g_table[G_DTYPES][G_BUCKETS][G_SMALL][G_KCLS];
+static bool g_table_init = [] {
+  for (int d = 0; d < G_DTYPES; ++d)
+    for (int b = 0; b < G_BUCKETS; ++b)
+      for (int s = 0; s < G_SMALL; ++s)
+        for (int q = 0; q < G_KCLS; ++q) g_table[d][b][s][q] = b >= 4 ? 1 : 0;
+  return true;
+}();
 
 static int bucket_of(size_t m, size_t n_real) {
   size_t d = std::max(m, n_real);
@@ -271,11 +280,17 @@ static int bucket_of(size_t m, size_t n_real) {
   b = std::min(std::max(b, 4), 11);
   return b - 4;
 }
+static int small_of(size_t m, size_t n_real) {
+  const size_t d = std::min(m, n_real);
+  return d < 64 ? 0 : (d < 128 ? 1 : 2);
+}
+static int kcls_of(size_t k_real) { return k_real < 128 ? 0 : (k_real < 512 ? 1 : 2); }
 
-int grouped_cfg(int dtype, size_t m, size_t n) {
-  const size_t n_real = (dtype == FERRAY_DTYPE_C64 || dtype == FERRAY_DTYPE_C32) ? 2 * n : n;
+int grouped_cfg(int dtype, size_t m, size_t n, size_t k) {
+  const bool cplx = dtype == FERRAY_DTYPE_C64 || dtype == FERRAY_DTYPE_C32;
+  const size_t n_real = cplx ? 2 * n : n, k_real = cplx ? 2 * k : k;
   std::lock_guard<std::mutex> lk(g_table_mu);
-  return g_table[dtype][bucket_of(m, n_real)];
+  return g_table[dtype][bucket_of(m, n_real)][small_of(m, n_real)][kcls_of(k_real)];
 }
 
 static size_t problem_tiles(int dtype, int cfg, size_t m, size_t n, int* tiles_n) {
@@ -371,7 +386,7 @@ static int build_desc(ferray_grouped_plan* plan, const ferray_gemm_problem* prob
     const ferray_gemm_problem& p = probs[order[j].second];
     GProblem& g = gp[j];
     g.dtype = p.dtype;
-    g.cfg = grouped_cfg(p.dtype, p.m, p.n);
+    g.cfg = grouped_cfg(p.dtype, p.m, p.n, p.k);
     const size_t nt = problem_tiles(p.dtype, g.cfg, p.m, p.n, &g.tiles_n);
     if (total + nt > plan->max_tiles || total + nt > 0x7fffffff) return FERRAY_ERR_INVALID;
     g.tile_start = (int)total;
@@ -438,22 +453,34 @@ using namespace fg;
 extern "C" {
 
 int ferray_set_tile_table(const int* table, size_t len) {
-  if (!table || len != (size_t)G_DTYPES * G_BUCKETS) return FERRAY_ERR_INVALID;
+  const bool legacy = len == (size_t)G_DTYPES * G_BUCKETS;
+  if (!table || (!legacy && len != (size_t)G_DTYPES * G_CELLS)) return FERRAY_ERR_INVALID;
+  for (size_t i = 0; i < len; ++i)
+    if (table[i] != 0 && table[i] != 1) return FERRAY_ERR_INVALID;
   std::lock_guard<std::mutex> lk(g_table_mu);
   for (int d = 0; d < G_DTYPES; ++d)
-    for (int b = 0; b < G_BUCKETS; ++b) {
-      const int v = table[d * G_BUCKETS + b];
-      if (v != 0 && v != 1) return FERRAY_ERR_INVALID;
-      g_table[d][b] = v;
-    }
+    for (int b = 0; b < G_BUCKETS; ++b)
+      for (int s = 0; s < G_SMALL; ++s)
+        for (int q = 0; q < G_KCLS; ++q)
+          g_table[d][b][s][q] = legacy ? table[d * G_BUCKETS + b]
+                                       : table[((d * G_BUCKETS + b) * G_SMALL + s) * G_KCLS + q];
   return FERRAY_OK;
 }
 
 int ferray_get_tile_table(int* table, size_t len) {
-  if (!table || len != (size_t)G_DTYPES * G_BUCKETS) return FERRAY_ERR_INVALID;
+  const bool legacy = len == (size_t)G_DTYPES * G_BUCKETS;
+  if (!table || (!legacy && len != (size_t)G_DTYPES * G_CELLS)) return FERRAY_ERR_INVALID;
   std::lock_guard<std::mutex> lk(g_table_mu);
   for (int d = 0; d < G_DTYPES; ++d)
-    for (int b = 0; b < G_BUCKETS; ++b) table[d * G_BUCKETS + b] = g_table[d][b];
+    for (int b = 0; b < G_BUCKETS; ++b) {
+      if (legacy) {
+        table[d * G_BUCKETS + b] = g_table[d][b][G_SMALL - 1][G_KCLS - 1];
+        continue;
+      }
+      for (int s = 0; s < G_SMALL; ++s)
+        for (int q = 0; q < G_KCLS; ++q)
+          table[((d * G_BUCKETS + b) * G_SMALL + s) * G_KCLS + q] = g_table[d][b][s][q];
+    }
   return FERRAY_OK;
 }
 
@@ -538,7 +565,7 @@ int ferray_gemm_grouped_dev(const ferray_gemm_problem* problems, size_t count, v
     if ((s = check_problem(problems[i]))) return s;
     const ferray_gemm_problem& p = problems[i];
     int tn;
-    if (p.m && p.n && p.k) tiles += problem_tiles(p.dtype, grouped_cfg(p.dtype, p.m, p.n), p.m, p.n, &tn);
+    if (p.m && p.n && p.k) tiles += problem_tiles(p.dtype, grouped_cfg(p.dtype, p.m, p.n, p.k), p.m, p.n, &tn);
   }
   // The per-thread plan lives until the thread ends; it is not released in a
   // destructor because thread teardown can run after the HIP runtime's own.

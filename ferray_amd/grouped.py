@@ -20,7 +20,7 @@ __all__ = ["GemmProblem", "Plan", "grouped", "problem", "load_tile_table", "tile
            "TABLE_PATH"]
 
 TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tile_table.json")
-N_DTYPES, N_BUCKETS = 5, 8
+N_DTYPES, N_BUCKETS, N_SMALL, N_KCLS = 5, 8, 3, 3
 
 
 def problem(dtype: int, m: int, n: int, k: int, a: int, b: int, c: int, alpha=1.0, beta=0.0,
@@ -74,14 +74,30 @@ class Plan:
             pass
 
 
-def tile_table() -> list[list[int]]:
-    buf = (C.c_int * (N_DTYPES * N_BUCKETS))()
-    check(lib().ferray_get_tile_table(C.cast(buf, C.c_void_p), len(buf)), "get_tile_table")
-    return [list(buf[d * N_BUCKETS:(d + 1) * N_BUCKETS]) for d in range(N_DTYPES)]
+def tile_table() -> list:
+    """The table as nested lists [dtype][size bucket][small-side class][k class]
+    (0 = 64x64 tiles, 1 = 128x128; see include/ferray_gemm.h)."""
+    n = N_DTYPES * N_BUCKETS * N_SMALL * N_KCLS
+    buf = (C.c_int * n)()
+    check(lib().ferray_get_tile_table(C.cast(buf, C.c_void_p), n), "get_tile_table")
+    flat = list(buf)
+    it = iter(flat)
+    return [[[[next(it) for _ in range(N_KCLS)] for _ in range(N_SMALL)]
+             for _ in range(N_BUCKETS)] for _ in range(N_DTYPES)]
+
+
+def _flatten(x):
+    if isinstance(x, (list, tuple)):
+        for y in x:
+            yield from _flatten(y)
+    else:
+        yield int(x)
 
 
 def set_tile_table(table) -> None:
-    flat = [int(v) for row in table for v in row]
+    """Installs a table: the full nesting of tile_table(), or [dtype][bucket]
+    (each entry then covers every small-side and k class of its bucket)."""
+    flat = list(_flatten(table))
     buf = (C.c_int * len(flat))(*flat)
     check(lib().ferray_set_tile_table(C.cast(buf, C.c_void_p), len(flat)), "set_tile_table")
 

@@ -315,9 +315,14 @@ int ferray_matmul_batched_dev(int dtype, size_t batch, size_t a_batch,
                               const void *a, const void *b, void *c,
                               void *stream);
 
-/* Tile-shape table of the grouped kernel: entry [dtype*8 + b] is 0 (64x64
- * tiles) or 1 (128x128) for problems with ceil(log2(max(m, n_real))) =
- * clamp(., 4, 11) = b + 4; written by the package's autotuner. */
+/* Tile-shape table of the grouped kernel, written by the package's
+ * autotuner: entry [((dtype*8 + b)*3 + s)*3 + q] (len 360) is 0 (64x64
+ * tiles) or 1 (128x128) for problems with clamp(ceil(log2(max(m, n_real))),
+ * 4, 11) = b + 4, min(m, n_real) in [0,64) / [64,128) / [128,..) for
+ * s = 0 / 1 / 2 and k_real in [0,128) / [128,512) / [512,..) for q = 0 / 1 / 2
+ * (n_real = 2n, k_real = 2k for complex).  A table of len 40, entry
+ * [dtype*8 + b], sets every s and q of its bucket; get with len 40 returns
+ * the s = 2, q = 2 entries. */
 int ferray_set_tile_table(const int *table, size_t len);
 int ferray_get_tile_table(int *table, size_t len);
 

@@ -153,7 +153,7 @@ def test_tile_shape_does_not_change_results(mix, cfg):
     saved = gg.tile_table()
     try:
         gg.set_tile_table([[cfg] * gg.N_BUCKETS for _ in range(gg.N_DTYPES)])
-        assert gg.tile_table()[2] == [cfg] * gg.N_BUCKETS
+        assert set(v for v in gg._flatten(gg.tile_table())) == {cfg}
         for c, got in zip(mix, _run(mix)):
             assert _same(got, c.single)
     finally:

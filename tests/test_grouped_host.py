@@ -31,9 +31,15 @@ def test_problem_builder():
 def test_tile_table_roundtrip_and_validation():
     saved = gg.tile_table()
     try:
+        full = [[[[(d + b + s + q) % 2 for q in range(gg.N_KCLS)] for s in range(gg.N_SMALL)]
+                 for b in range(gg.N_BUCKETS)] for d in range(gg.N_DTYPES)]
+        gg.set_tile_table(full)
+        assert gg.tile_table() == full
+        # the per-bucket form covers every small-side and k class of its bucket
         t = [[(d + b) % 2 for b in range(gg.N_BUCKETS)] for d in range(gg.N_DTYPES)]
         gg.set_tile_table(t)
-        assert gg.tile_table() == t
+        assert gg.tile_table() == [[[[t[d][b]] * gg.N_KCLS] * gg.N_SMALL
+                                    for b in range(gg.N_BUCKETS)] for d in range(gg.N_DTYPES)]
         with pytest.raises(Exception):
             gg.set_tile_table([[2] * gg.N_BUCKETS for _ in range(gg.N_DTYPES)])
         with pytest.raises(Exception):
