@@ -62,8 +62,11 @@ def test_single_leaf_bit_exact(gpu, lower, dt, m, n):
 # Solves whose order differs from the reference's substitution: the blocked
 # recursion (m > 64), and upper single-leaf solves with 33 <= m <= 64, which
 # the reference splits at 32 rows and the device solves in one leaf.
+# Diagonal blocks of 65 to 256 rows run as one kernel (16-row leaves, 16
+# columns per workgroup): whole and ragged leaves and column groups.
 BLOCKED = ([(lower, m, n) for lower in (True, False)
-            for m, n in [(65, 4), (80, 7), (300, 50), (1000, 33), (517, 1)]] +
+            for m, n in [(65, 4), (80, 7), (300, 50), (1000, 33), (517, 1), (97, 3), (129, 16),
+                         (255, 40), (256, 1000), (2048, 700)]] +
            [(False, m, n) for m, n in [(33, 5), (40, 300), (64, 16500), (64, 1)]])
 
 
