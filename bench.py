@@ -197,20 +197,28 @@ def consumers(fa, steps: int, warmup: int):
     b0 = fa.DeviceBuffer(m * n * 8)
     L.ferray_dev_copy(b0.ptr, dB.ptr, m * n * 8, None)
 
-    def trsm_step():
+    def trsm_once():
+        """One solve from a fresh B, timed alone (the reset copy is outside
+        the events)."""
         L.ferray_dev_copy(dB.ptr, b0.ptr, m * n * 8, None)
+        e0, e1 = fa.Event(), fa.Event()
+        e0.record()
         st = L.ferray_trsm_dev(0, 0, m, n, 1.0, dA.ptr, m, dB.ptr, n, False, None)
+        e1.record()
         if st != 0:
             raise RuntimeError(f"ferray_trsm_dev status {st}")
+        fa.device_sync()
+        return e0.elapsed_ms(e1)
 
-    def copy_step():
-        L.ferray_dev_copy(dB.ptr, b0.ptr, m * n * 8, None)
-
-    ms = time_steps(fa, trsm_step, max(2, steps // 4), 1)
-    ms_copy = time_steps(fa, copy_step, 3, 1)
-    t = ms - ms_copy
+    for _ in range(3):
+        trsm_once()
+    runs = [trsm_once() for _ in range(7)]
+    t = float(np.median(runs))
     out["trsm_lower_f64_8192x8192"] = {"ms": round(t, 3), "tflops": round(m * m * n / (t * 1e-3) / 1e12, 2),
-                                       "note": "m^2 n flop; the B reset copy is timed separately and subtracted"}
+                                       "min_ms": round(min(runs), 3), "max_ms": round(max(runs), 3),
+                                       "note": "m^2 n flop; median of 7 solves after 3 warm-ups, each "
+                                               "timed alone with events on the library stream (the B "
+                                               "reset copy runs before the first event)"}
     log(f"trsm f64 8192x8192: {t:.2f} ms, {m * m * n / (t * 1e-3) / 1e12:.1f} TFLOP/s")
     for buf in (dA, dB, b0):
         buf.free()
