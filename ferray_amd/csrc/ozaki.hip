@@ -148,12 +148,14 @@ __device__ __forceinline__ void block_stats(Ssq acc, int nonfinite, int g, int* 
 
 // Column statistics of B (K x N row-major): partial pairs per k-slice
 // (gridDim.y slices), combined by oz_col_finish.
-__global__ void oz_col_partial(const double* __restrict__ b, int K, int N, int kslice,
-                               int* __restrict__ pe, double* __restrict__ ps,
-                               int* __restrict__ bad) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void oz_col_partial_block(int bx, int by, const double* __restrict__ b,
+                                                     int K, int N, int kslice,
+                                                     int* __restrict__ pe,
+                                                     double* __restrict__ ps,
+                                                     int* __restrict__ bad) {
+  const int j = bx * blockDim.x + threadIdx.x;
   if (j >= N) return;
-  const int k0 = blockIdx.y * kslice, k1 = min(K, k0 + kslice);
+  const int k0 = by * kslice, k1 = min(K, k0 + kslice);
   Ssq acc4[4] = {{INT_MIN, 0.0}, {INT_MIN, 0.0}, {INT_MIN, 0.0}, {INT_MIN, 0.0}};
   int nonfinite = 0;
   int k = k0;
@@ -173,9 +175,14 @@ __global__ void oz_col_partial(const double* __restrict__ b, int K, int N, int k
     acc4[0] = ssq_add(acc4[0], x);
   }
   const Ssq acc = ssq_merge(ssq_merge(acc4[0], acc4[1]), ssq_merge(acc4[2], acc4[3]));
-  pe[(size_t)blockIdx.y * N + j] = acc.E;
-  ps[(size_t)blockIdx.y * N + j] = acc.S;
+  pe[(size_t)by * N + j] = acc.E;
+  ps[(size_t)by * N + j] = acc.S;
   if (nonfinite) atomicOr(bad + j, 1);
+}
+__global__ void oz_col_partial(const double* __restrict__ b, int K, int N, int kslice,
+                               int* __restrict__ pe, double* __restrict__ ps,
+                               int* __restrict__ bad) {
+  oz_col_partial_block(blockIdx.x, blockIdx.y, b, K, N, kslice, pe, ps, bad);
 }
 
 __global__ void oz_col_finish(int N, int slices, int g, const int* __restrict__ pe,
@@ -228,10 +235,12 @@ __device__ __forceinline__ int sym_residue(const Limbs& L, const OzConsts& c, in
 
 // Residue bytes of 4 values modulo c.m[l], packed little-endian.
 __device__ __forceinline__ unsigned residue_word(const Limbs (&L)[4], const OzConsts& c, int l) {
-  unsigned v = 0;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) v |= ((unsigned)sym_residue(L[e], c, l) & 0xffu) << (8 * e);
-  return v;
+  const unsigned r0 = (unsigned)sym_residue(L[0], c, l), r1 = (unsigned)sym_residue(L[1], c, l);
+  const unsigned r2 = (unsigned)sym_residue(L[2], c, l), r3 = (unsigned)sym_residue(L[3], c, l);
+  // byte 0 of each, in order: two byte pairs, then the pairs' low halves
+  const unsigned lo = __builtin_amdgcn_perm(r1, r0, 0x0c0c0400u);
+  const unsigned hi = __builtin_amdgcn_perm(r3, r2, 0x0c0c0400u);
+  return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 
 // Row scale and residue planes of A in one kernel: one block per row of the
@@ -285,23 +294,22 @@ __global__ __launch_bounds__(256) void oz_scale_residues_a(const double* __restr
   }
 }
 
-// The same for rows of at most 8192 elements (Kp <= 8192), reading A from
+// The same for one row of at most 8192 elements (Kp <= 8192), reading A from
 // memory once: thread t holds k = 4t + 1024 g + e (g < 8, e < 4) of its row
 // in registers from the norm to the residues.
 template <int NM>
-__global__ __launch_bounds__(256) void oz_scale_residues_a_reg(const double* __restrict__ a, int M,
-                                                               int K, int Kp, int vec, int g,
-                                                               int* __restrict__ shift,
-                                                               int* __restrict__ bad,
-                                                               uint8_t* __restrict__ planes,
-                                                               size_t plane_bytes) {
+__device__ __forceinline__ void oz_scale_residues_a_row(int row, const double* __restrict__ a,
+                                                        int M, int K, int Kp, int vec, int g,
+                                                        int* __restrict__ shift,
+                                                        int* __restrict__ bad,
+                                                        uint8_t* __restrict__ planes,
+                                                        size_t plane_bytes) {
   constexpr int G = 8;
   const OzConsts& c = g_oz[NM];
   __shared__ int pe[256];
   __shared__ double ps[256];
   __shared__ int anybad;
   if (threadIdx.x == 0) anybad = 0;
-  const int row = blockIdx.x;
   const double* src_row = a + (size_t)row * K;
   double x[G][4];
 #pragma unroll
@@ -364,16 +372,17 @@ __global__ __launch_bounds__(256) void oz_scale_residues_a_reg(const double* __r
 // B^T planes: plane l is Np x Kp int8 (row n holds column n of B).  A block
 // takes 128 k x 32 n of B: coalesced row loads, stored transposed in LDS;
 // then each thread turns 4 consecutive k of one n into residues, so a wave's
-// 4-byte stores cover two full 128-byte rows of a plane.  Grid: Np/32 x Kp/128.
+// 4-byte stores cover two full 128-byte rows of a plane.  Tiles: Np/32 x Kp/128.
 template <int NM>
-__global__ __launch_bounds__(256) void oz_residues_bt(const double* __restrict__ b, int K, int N,
-                                                      int Kp, const int* __restrict__ shift,
-                                                      uint8_t* __restrict__ planes,
-                                                      size_t plane_bytes) {
+__device__ __forceinline__ void oz_residues_bt_tile(int tn_, int tk_, const double* __restrict__ b,
+                                                    int K, int N, int Kp,
+                                                    const int* __restrict__ shift,
+                                                    uint8_t* __restrict__ planes,
+                                                    size_t plane_bytes) {
   const OzConsts& c = g_oz[NM];
   __shared__ double tile[32][130];
   const int tid = threadIdx.x;
-  const int n0 = blockIdx.x * 32, k0 = blockIdx.y * 128;
+  const int n0 = tn_ * 32, k0 = tk_ * 128;
   {
     const int nl = tid & 31, n = n0 + nl;
     for (int kr = tid >> 5; kr < 128; kr += 8) {
@@ -402,6 +411,36 @@ __global__ __launch_bounds__(256) void oz_residues_bt(const double* __restrict__
     for (int j = 0; j < 4; ++j)
       *reinterpret_cast<unsigned*>(dst + (size_t)(8 * j) * Kp + l * plane_bytes) =
           residue_word(L[j], c, l);
+}
+
+// A's scale-and-residue pass and the column statistics of B in one launch:
+// block b does A row ia(b) or column block ib(b) of B's statistics, the A
+// rows spread evenly through the grid, so that the A side's arithmetic and
+// the B scan's memory traffic run on the chip at the same time.  Each block
+// does exactly what the separate passes did (same values).
+template <int NM>
+__global__ __launch_bounds__(256) void oz_residues_a_colstats(
+    const double* __restrict__ a, int M, int K, int Kp, int vec, int g, int* __restrict__ shift_a,
+    int* __restrict__ bad_a, uint8_t* __restrict__ planes_a, size_t a_plane, int n_a,
+    const double* __restrict__ b, int N, int kslice, int n_cx, int n_c, int* __restrict__ pe,
+    double* __restrict__ ps, int* __restrict__ bad_b) {
+  const long long total = (long long)n_a + n_c, blk = blockIdx.x;
+  const long long before = blk * n_a / total, upto = (blk + 1) * n_a / total;
+  if (upto > before) {
+    oz_scale_residues_a_row<NM>((int)before, a, M, K, Kp, vec, g, shift_a, bad_a, planes_a,
+                                a_plane);
+  } else {
+    const int t = (int)(blk - upto);
+    oz_col_partial_block(t % n_cx, t / n_cx, b, K, N, kslice, pe, ps, bad_b);
+  }
+}
+
+template <int NM>
+__global__ __launch_bounds__(256) void oz_residues_bt(const double* __restrict__ b, int K, int N,
+                                                      int Kp, const int* __restrict__ shift,
+                                                      uint8_t* __restrict__ planes,
+                                                      size_t plane_bytes) {
+  oz_residues_bt_tile<NM>(blockIdx.x, blockIdx.y, b, K, N, Kp, shift, planes, plane_bytes);
 }
 
 // GEMM epilogue: residue of the exact i32 product modulo m_plane, as a byte.
@@ -662,22 +701,27 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
 
   const unsigned gn = (unsigned)cdiv(n, 256);
   hipLaunchKernelGGL(oz_init, dim3(gn), dim3(256), 0, st, badB, (int)n);
-  const dim3 cgrid(gn, (unsigned)slices);
-  hipLaunchKernelGGL(oz_col_partial, cgrid, dim3(256), 0, st, b, (int)k, (int)n, kslice, pe, ps,
-                     badB);
-  hipLaunchKernelGGL(oz_col_finish, dim3(gn), dim3(256), 0, st, (int)n, (int)slices, gamma, pe,
-                     ps, tB);
-  FG_CHECK(hipGetLastError());
-
   const int vec_a = (k % 2 == 0) && ((uintptr_t)a % 16 == 0);
+  const size_t n_c = (size_t)gn * slices;
+  if (Mp + n_c > 0x7fffffff) return FERRAY_ERR_INVALID;
   with_moduli(nmod, [&](auto nm) {
     constexpr int NM = decltype(nm)::value;
-    if (Kp <= 8192)
-      hipLaunchKernelGGL((oz_scale_residues_a_reg<NM>), dim3((unsigned)Mp), dim3(256), 0, st, a,
-                         (int)m, (int)k, (int)Kp, vec_a, gamma, sA, badA, ws + o_pa, a_plane);
-    else
+    if (Kp <= 8192) {
+      // A's pass beside B's column statistics (rows of at most 8192)
+      hipLaunchKernelGGL((oz_residues_a_colstats<NM>), dim3((unsigned)(Mp + n_c)), dim3(256), 0,
+                         st, a, (int)m, (int)k, (int)Kp, vec_a, gamma, sA, badA, ws + o_pa,
+                         a_plane, (int)Mp, b, (int)n, kslice, (int)gn, (int)n_c, pe, ps, badB);
+    } else {
+      hipLaunchKernelGGL(oz_col_partial, dim3(gn, (unsigned)slices), dim3(256), 0, st, b, (int)k,
+                         (int)n, kslice, pe, ps, badB);
       hipLaunchKernelGGL((oz_scale_residues_a<NM>), dim3((unsigned)Mp), dim3(256), 0, st, a,
                          (int)m, (int)k, (int)Kp, vec_a, gamma, sA, badA, ws + o_pa, a_plane);
+    }
+  });
+  hipLaunchKernelGGL(oz_col_finish, dim3(gn), dim3(256), 0, st, (int)n, (int)slices, gamma, pe,
+                     ps, tB);
+  with_moduli(nmod, [&](auto nm) {
+    constexpr int NM = decltype(nm)::value;
     hipLaunchKernelGGL((oz_residues_bt<NM>), dim3((unsigned)(Np / 32), (unsigned)(Kp / 128)),
                        dim3(256), 0, st, b, (int)k, (int)n, (int)Kp, tB, ws + o_pb, b_plane);
   });
