@@ -249,8 +249,8 @@ __device__ __forceinline__ unsigned residue_word(const Limbs (&L)[4], const OzCo
 // A is read from HBM once.  Rows at or beyond M are zero padding.
 template <int NM>
 __global__ __launch_bounds__(256) void oz_scale_residues_a(const double* __restrict__ a, int M,
-                                                           int K, int Kp, int vec, int g,
-                                                           int* __restrict__ shift,
+                                                           int K, int Kp, int ldp, int vec,
+                                                           int g, int* __restrict__ shift,
                                                            int* __restrict__ bad,
                                                            uint8_t* __restrict__ planes,
                                                            size_t plane_bytes) {
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void oz_scale_residues_a(const double* __restr
     }
   }
   const double* src_row = a + (size_t)row * K;
-  uint8_t* dst_row = planes + (size_t)row * Kp;
+  uint8_t* dst_row = planes + (size_t)row * ldp;
   for (int k0 = 4 * threadIdx.x; k0 < Kp; k0 += 4 * 256) {
     double x[4] = {0.0, 0.0, 0.0, 0.0};
     if (row < M) {
@@ -299,8 +299,8 @@ __global__ __launch_bounds__(256) void oz_scale_residues_a(const double* __restr
 // in registers from the norm to the residues.
 template <int NM>
 __device__ __forceinline__ void oz_scale_residues_a_row(int row, const double* __restrict__ a,
-                                                        int M, int K, int Kp, int vec, int g,
-                                                        int* __restrict__ shift,
+                                                        int M, int K, int Kp, int ldp,
+                                                        int vec, int g, int* __restrict__ shift,
                                                         int* __restrict__ bad,
                                                         uint8_t* __restrict__ planes,
                                                         size_t plane_bytes) {
@@ -354,7 +354,7 @@ __device__ __forceinline__ void oz_scale_residues_a_row(int row, const double* _
     bad[row] = bd;
   }
   if (row >= M) sh = 0;
-  uint8_t* dst_row = planes + (size_t)row * Kp;
+  uint8_t* dst_row = planes + (size_t)row * ldp;
 #pragma unroll
   for (int gi = 0; gi < G; ++gi) {
     const int k0 = 4 * threadIdx.x + 1024 * gi;
@@ -375,7 +375,7 @@ __device__ __forceinline__ void oz_scale_residues_a_row(int row, const double* _
 // 4-byte stores cover two full 128-byte rows of a plane.  Tiles: Np/32 x Kp/128.
 template <int NM>
 __device__ __forceinline__ void oz_residues_bt_tile(int tn_, int tk_, const double* __restrict__ b,
-                                                    int K, int N, int Kp,
+                                                    int K, int N, int Kp, int ldp,
                                                     const int* __restrict__ shift,
                                                     uint8_t* __restrict__ planes,
                                                     size_t plane_bytes) {
@@ -404,12 +404,12 @@ __device__ __forceinline__ void oz_residues_bt_tile(int tn_, int tk_, const doub
       L[j][e] = limbs_of(rint(ldexp(isfinite(x) ? x : 0.0, sh)));
     }
   }
-  uint8_t* dst = planes + (size_t)(n0 + tn) * Kp + k0 + 4 * tk;
+  uint8_t* dst = planes + (size_t)(n0 + tn) * ldp + k0 + 4 * tk;
 #pragma unroll
   for (int l = 0; l < NM; ++l)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      *reinterpret_cast<unsigned*>(dst + (size_t)(8 * j) * Kp + l * plane_bytes) =
+      *reinterpret_cast<unsigned*>(dst + (size_t)(8 * j) * ldp + l * plane_bytes) =
           residue_word(L[j], c, l);
 }
 
@@ -420,15 +420,16 @@ __device__ __forceinline__ void oz_residues_bt_tile(int tn_, int tk_, const doub
 // does exactly what the separate passes did (same values).
 template <int NM>
 __global__ __launch_bounds__(256) void oz_residues_a_colstats(
-    const double* __restrict__ a, int M, int K, int Kp, int vec, int g, int* __restrict__ shift_a,
+    const double* __restrict__ a, int M, int K, int Kp, int ldp, int vec, int g,
+    int* __restrict__ shift_a,
     int* __restrict__ bad_a, uint8_t* __restrict__ planes_a, size_t a_plane, int n_a,
     const double* __restrict__ b, int N, int kslice, int n_cx, int n_c, int* __restrict__ pe,
     double* __restrict__ ps, int* __restrict__ bad_b) {
   const long long total = (long long)n_a + n_c, blk = blockIdx.x;
   const long long before = blk * n_a / total, upto = (blk + 1) * n_a / total;
   if (upto > before) {
-    oz_scale_residues_a_row<NM>((int)before, a, M, K, Kp, vec, g, shift_a, bad_a, planes_a,
-                                a_plane);
+    oz_scale_residues_a_row<NM>((int)before, a, M, K, Kp, ldp, vec, g, shift_a, bad_a,
+                                planes_a, a_plane);
   } else {
     const int t = (int)(blk - upto);
     oz_col_partial_block(t % n_cx, t / n_cx, b, K, N, kslice, pe, ps, bad_b);
@@ -437,10 +438,11 @@ __global__ __launch_bounds__(256) void oz_residues_a_colstats(
 
 template <int NM>
 __global__ __launch_bounds__(256) void oz_residues_bt(const double* __restrict__ b, int K, int N,
-                                                      int Kp, const int* __restrict__ shift,
+                                                      int Kp, int ldp,
+                                                      const int* __restrict__ shift,
                                                       uint8_t* __restrict__ planes,
                                                       size_t plane_bytes) {
-  oz_residues_bt_tile<NM>(blockIdx.x, blockIdx.y, b, K, N, Kp, shift, planes, plane_bytes);
+  oz_residues_bt_tile<NM>(blockIdx.x, blockIdx.y, b, K, N, Kp, ldp, shift, planes, plane_bytes);
 }
 
 // GEMM epilogue: residue of the exact i32 product modulo m_plane, as a byte.
@@ -674,10 +676,14 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
   int s = make_consts(nmod, cst, gamma);
   if (s) return s;
   if ((s = oz_tables())) return s;
-  // Kp: a whole number of GEMM stages and of the 128-k residue tiles.
+  // Kp: a whole number of GEMM stages and of the 128-k residue tiles.  ldp:
+  // the planes' row pitch, 128 bytes past Kp when Kp is a multiple of 1024
+  // (rows a power-of-two multiple of 1 KiB apart cost the 256 x 256 main
+  // loop 7 % at 8192, measured against pitches of K + 128 .. K + 512).
   const size_t Kp = align_up(k, 128), Mp = align_up(m, S::BM), Np = align_up(n, S::BN);
+  const size_t ldp = Kp % 1024 == 0 ? Kp + 128 : Kp;
   const size_t ldr = align_up(n, 8);
-  const size_t a_plane = Mp * Kp, b_plane = Np * Kp, r_plane = align_up(m * ldr, 256);
+  const size_t a_plane = Mp * ldp, b_plane = Np * ldp, r_plane = align_up(m * ldr, 256);
   const int kslice = 256;
   const size_t slices = cdiv(k, kslice);
   size_t off = 0;
@@ -709,13 +715,15 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
     if (Kp <= 8192) {
       // A's pass beside B's column statistics (rows of at most 8192)
       hipLaunchKernelGGL((oz_residues_a_colstats<NM>), dim3((unsigned)(Mp + n_c)), dim3(256), 0,
-                         st, a, (int)m, (int)k, (int)Kp, vec_a, gamma, sA, badA, ws + o_pa,
-                         a_plane, (int)Mp, b, (int)n, kslice, (int)gn, (int)n_c, pe, ps, badB);
+                         st, a, (int)m, (int)k, (int)Kp, (int)ldp, vec_a, gamma, sA, badA,
+                         ws + o_pa, a_plane, (int)Mp, b, (int)n, kslice, (int)gn, (int)n_c, pe,
+                         ps, badB);
     } else {
       hipLaunchKernelGGL(oz_col_partial, dim3(gn, (unsigned)slices), dim3(256), 0, st, b, (int)k,
                          (int)n, kslice, pe, ps, badB);
       hipLaunchKernelGGL((oz_scale_residues_a<NM>), dim3((unsigned)Mp), dim3(256), 0, st, a,
-                         (int)m, (int)k, (int)Kp, vec_a, gamma, sA, badA, ws + o_pa, a_plane);
+                         (int)m, (int)k, (int)Kp, (int)ldp, vec_a, gamma, sA, badA, ws + o_pa,
+                         a_plane);
     }
   });
   hipLaunchKernelGGL(oz_col_finish, dim3(gn), dim3(256), 0, st, (int)n, (int)slices, gamma, pe,
@@ -723,7 +731,8 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
   with_moduli(nmod, [&](auto nm) {
     constexpr int NM = decltype(nm)::value;
     hipLaunchKernelGGL((oz_residues_bt<NM>), dim3((unsigned)(Np / 32), (unsigned)(Kp / 128)),
-                       dim3(256), 0, st, b, (int)k, (int)n, (int)Kp, tB, ws + o_pb, b_plane);
+                       dim3(256), 0, st, b, (int)k, (int)n, (int)Kp, (int)ldp, tB, ws + o_pb,
+                       b_plane);
   });
   FG_CHECK(hipGetLastError());
 
@@ -732,8 +741,8 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
   p.bt = ws + o_pb;
   p.a_plane = a_plane;
   p.b_plane = b_plane;
-  p.lda = Kp;
-  p.ldb = Kp;
+  p.lda = ldp;
+  p.ldb = ldp;
   p.kbytes = (int)Kp;
   p.tiles_m = (int)(Mp / S::BM);
   p.tiles_n = (int)(Np / S::BN);
