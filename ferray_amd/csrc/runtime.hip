@@ -8,6 +8,9 @@
 #include <mutex>
 #include <unordered_map>
 
+#include <sys/syscall.h>
+#include <unistd.h>
+
 #include "common.hpp"
 
 namespace fg {
@@ -19,8 +22,12 @@ int g_device = 0;
 
 struct ThreadState {
   hipStream_t stream = nullptr;
+  // A thread that ends releases its stream and workspaces.  The main
+  // thread's state goes with the process: its destructor runs while the
+  // process tears down, when the runtime (or a tool layered on it) may
+  // already be gone, so it makes no HIP calls.
   ~ThreadState() {
-    if (stream) {
+    if (stream && syscall(SYS_gettid) != getpid()) {
       release_workspace(stream);
       (void)hipStreamDestroy(stream);
     }
