@@ -10,7 +10,7 @@
 //      |(A'B')_ij| <= M/4, so A'B' is recovered exactly from its residues.
 //   2. Residues (FP64 -> INT8).  For each modulus m_l (pairwise coprime,
 //      <= 256) the symmetric residues of A' and B' fit int8.
-//   3. N INT8 GEMMs on v_mfma_i32_32x32x32_i8 (exact: |sum| <= k*2^14 < 2^31
+//   3. N INT8 GEMMs on the INT8 matrix cores (exact: |sum| <= k*2^14 < 2^31
 //      for k <= 2^16 per chunk), each reduced mod m_l in the epilogue and
 //      stored as one byte.
 //   4. Reconstruction (FP64).  Chinese remaindering with the weights
@@ -466,8 +466,8 @@ struct EpiResidue {
 };
 
 // Reconstruction: C = alpha * 2^-(s_i+t_j) * CRT(r_1..r_N) + beta * C.  Each
-// thread finishes 8 consecutive columns of one row (residue planes have a
-// row pitch of ldr >= N, a multiple of 8, so one 8-byte load per plane).
+// thread finishes 4 consecutive columns of one row (residue planes have a
+// row pitch of ldr >= N, a multiple of 8, so one 4-byte load per plane).
 // Number of radix pieces the CRT weights are split into: 3 while
 // NM * 255 * 2^R stays below 2^52 with 3R >= log2(M) (up to 15 moduli), else
 // 4 pieces of 32 bits.
@@ -506,11 +506,14 @@ __device__ __forceinline__ double u32_to_double(unsigned u) {
   return __hiloint2double(0x43300000, (int)u) - 4503599627370496.0;
 }
 
-// Each wave finishes 512 consecutive columns of one row.  Lane L first
-// reconstructs columns 8L .. 8L+7 from one 8-byte load per residue plane;
-// the values are then exchanged through LDS so that every store (and every
-// read of C when beta != 0) covers 64 consecutive doubles.
-template <int NM>
+// Each wave finishes 256 consecutive columns of one row: lane L reconstructs
+// columns 4L .. 4L+3 from one 4-byte load per residue plane and stores them
+// itself (two 16-byte stores when PAIRS: C's rows 16-byte aligned).  Four
+// columns per lane keep the kernel at ~60 registers, so enough waves are
+// resident to cover the residue loads; 8 columns per lane with the stores
+// regrouped through LDS ran 18-24 % slower at 8192^2 and 14 moduli, with
+// identical results.
+template <int NM, bool PAIRS>
 __global__ __launch_bounds__(256) void oz_reconstruct(const uint8_t* __restrict__ res,
                                                       size_t plane_bytes, int ldr, int M, int N,
                                                       const int* __restrict__ sA,
@@ -519,63 +522,64 @@ __global__ __launch_bounds__(256) void oz_reconstruct(const uint8_t* __restrict_
                                                       const int* __restrict__ badB, double alpha,
                                                       double beta, double* __restrict__ c,
                                                       OzConsts k) {
-  __shared__ double xch[4][512];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int segs = (N + 511) / 512;
+  const int segs = (N + 255) / 256;
   const size_t unit = (size_t)blockIdx.x * 4 + w;
-  const bool live = unit < (size_t)M * segs;
-  const int i = live ? (int)(unit / segs) : 0;
-  const int base = live ? (int)(unit % segs) * 512 : 0;
-  const int j0 = base + 8 * lane;
-  double v[8];
+  if (unit >= (size_t)M * segs) return;
+  const int i = (int)(unit / segs);
+  const int j0 = (int)(unit % segs) * 256 + 4 * lane;
+  if (j0 >= N) return;
+  // Piece sums of the 4 columns, accumulated modulus by modulus.  All
+  // residue words are requested at once (the weights are kernel arguments:
+  // measured faster than constant-memory or LDS copies here).
+  const uint8_t* src = res + (size_t)i * ldr + j0;
+  constexpr int P = CrtPieces<NM>::value;
+  double S[4][P];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) v[e] = 0.0;
-  if (live && j0 < N) {
-    const uint8_t* src = res + (size_t)i * ldr + j0;
-    // Piece sums of the 8 columns, accumulated modulus by modulus.
-    constexpr int P = CrtPieces<NM>::value;
-    double S[8][P];
+  for (int e = 0; e < 4; ++e)
 #pragma unroll
-    for (int e = 0; e < 8; ++e)
+    for (int q = 0; q < P; ++q) S[e][q] = 0.0;
+  unsigned pk[NM];
 #pragma unroll
-      for (int q = 0; q < P; ++q) S[e][q] = 0.0;
-    // All residue words are requested at once (the weights are kernel
-    // arguments: measured faster than constant-memory or LDS copies here).
-    unsigned long long pk[NM];
+  for (int l = 0; l < NM; ++l) pk[l] = *reinterpret_cast<const unsigned*>(src + (size_t)l * plane_bytes);
 #pragma unroll
-    for (int l = 0; l < NM; ++l)
-      pk[l] = *reinterpret_cast<const unsigned long long*>(src + (size_t)l * plane_bytes);
+  for (int l = 0; l < NM; ++l) {
 #pragma unroll
-    for (int l = 0; l < NM; ++l) {
+    for (int e = 0; e < 4; ++e) {
+      const double r = u32_to_double((pk[l] >> (8 * e)) & 0xffu);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const double r = u32_to_double((unsigned)(pk[l] >> (8 * e)) & 0xffu);
-#pragma unroll
-        for (int q = 0; q < P; ++q) S[e][q] = fma(r, k.w[l][q], S[e][q]);
-      }
-    }
-    const int si = sA[i], bi = badA[i];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int j = j0 + e;
-      if (j >= N) break;
-      double x = ldexp(crt_finish<P>(S[e], k), -(si + tB[j]));
-      if (bi | badB[j]) x = __builtin_nan("");
-      v[e] = x;
+      for (int q = 0; q < P; ++q) S[e][q] = fma(r, k.w[l][q], S[e][q]);
     }
   }
+  const int si = sA[i], bi = badA[i];
+  double v[4];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) xch[w][8 * lane + e] = v[e];
-  __syncthreads();
-  if (!live) return;
-  double* row = c + (size_t)i * N;
+  for (int e = 0; e < 4; ++e) {
+    const int j = min(j0 + e, N - 1);  // columns past N are computed, never stored
+    double x = ldexp(crt_finish<P>(S[e], k), -(si + tB[j]));
+    if (bi | badB[j]) x = __builtin_nan("");
+    v[e] = x * alpha;
+  }
+  double* row = c + (size_t)i * N + j0;
+  if (PAIRS && j0 + 4 <= N) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int j = base + lane + 64 * q;
-    if (j >= N) break;
-    double x = xch[w][lane + 64 * q] * alpha;
-    if (beta != 0.0) x = fma(beta, row[j], x);
-    row[j] = x;
+    for (int h = 0; h < 2; ++h) {
+      d2 x = d2{v[2 * h], v[2 * h + 1]};
+      if (beta != 0.0) {
+        const d2 o = *reinterpret_cast<const d2*>(row + 2 * h);
+        x = d2{fma(beta, o[0], x[0]), fma(beta, o[1], x[1])};
+      }
+      *reinterpret_cast<d2*>(row + 2 * h) = x;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (j0 + e >= N) break;
+      double x = v[e];
+      if (beta != 0.0) x = fma(beta, row[e], x);
+      row[e] = x;
+    }
   }
 }
 
@@ -758,13 +762,19 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
   }
   if ((s = launch_mfma_nt<ELEM_I8, S, false>(p, nmod, e, st))) return s;
 
-  const size_t units = m * cdiv(n, 512);  // one wave per 512 columns of a row
+  const size_t units = m * cdiv(n, 256);  // one wave per 256 columns of a row
   if (cdiv(units, 4) > 0x7fffffff) return FERRAY_ERR_INVALID;
+  const bool pairs = n % 2 == 0 && (uintptr_t)c % 16 == 0;
   with_moduli(nmod, [&](auto nm) {
     constexpr int NM = decltype(nm)::value;
-    hipLaunchKernelGGL((oz_reconstruct<NM>), dim3((unsigned)cdiv(units, 4)), dim3(256), 0, st,
-                       ws + o_r, r_plane, (int)ldr, (int)m, (int)n, sA, tB, badA, badB, alpha,
-                       beta, c, cst);
+    if (pairs)
+      hipLaunchKernelGGL((oz_reconstruct<NM, true>), dim3((unsigned)cdiv(units, 4)), dim3(256), 0,
+                         st, ws + o_r, r_plane, (int)ldr, (int)m, (int)n, sA, tB, badA, badB,
+                         alpha, beta, c, cst);
+    else
+      hipLaunchKernelGGL((oz_reconstruct<NM, false>), dim3((unsigned)cdiv(units, 4)), dim3(256), 0,
+                         st, ws + o_r, r_plane, (int)ldr, (int)m, (int)n, sA, tB, badA, badB,
+                         alpha, beta, c, cst);
   });
   FG_CHECK(hipGetLastError());
   return FERRAY_OK;
