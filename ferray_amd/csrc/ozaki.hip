@@ -446,6 +446,27 @@ __global__ __launch_bounds__(256) void oz_residues_bt(const double* __restrict__
 }
 
 // GEMM epilogue: residue of the exact i32 product modulo m_plane, as a byte.
+// |acc| < 2^31: the product with the rounded reciprocal is within 2^-21 of
+// acc/m, so the floor is off by at most one and fixed below.
+__device__ __forceinline__ uint8_t residue_byte(int acc, int mm, double inv) {
+  const double q = floor((double)acc * inv);
+  int v = acc - (int)q * mm;
+  if (v < 0) v += mm;
+  if (v >= mm) v -= mm;
+  return (uint8_t)v;
+}
+
+// EpiResidue for a block tile inside C: one plane (the block's), no bounds
+// checks (measured 5 % off the 14-moduli product at 8192^3).
+struct EpiResidueFast {
+  uint8_t* plane;
+  int ld, mm;
+  double inv;
+  __device__ __forceinline__ void operator()(int row, int col, int, int acc) const {
+    plane[(size_t)row * ld + col] = residue_byte(acc, mm, inv);
+  }
+};
+
 struct EpiResidue {
   uint8_t* r;
   size_t plane_bytes;
@@ -454,14 +475,17 @@ struct EpiResidue {
   double inv_m[OZ_MAX_MODULI];
   __device__ __forceinline__ void operator()(int row, int col, int z, int acc) const {
     if (row >= M || col >= N) return;
-    const int mm = m[z];
-    // |acc| < 2^31: the product with the rounded reciprocal is within 2^-21
-    // of acc/m, so the floor is off by at most one and fixed below.
-    const double q = floor((double)acc * inv_m[z]);
-    int v = acc - (int)q * mm;
-    if (v < 0) v += mm;
-    if (v >= mm) v -= mm;
-    r[(size_t)z * plane_bytes + (size_t)row * ld + col] = (uint8_t)v;
+    r[(size_t)z * plane_bytes + (size_t)row * ld + col] = residue_byte(acc, m[z], inv_m[z]);
+  }
+  __device__ __forceinline__ bool interior(int r0, int c0, int rows, int cols) const {
+    return r0 + rows <= M && c0 + cols <= N;
+  }
+  // The NT kernel multiplies plane pair z = blockIdx.y, so every element a
+  // block stores belongs to that plane.
+  template <class F>
+  __device__ __forceinline__ void with_fast(F&& f) const {
+    const int z = blockIdx.y;
+    f(EpiResidueFast{r + (size_t)z * plane_bytes, ld, m[z], inv_m[z]});
   }
 };
 
