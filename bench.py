@@ -407,14 +407,18 @@ def run_sweep(fa, steps: int, warmup: int):
         if st != 0:
             out[name] = {"error": f"status {st}"}
             continue
-        ms = time_steps(fa, fn, max(3, steps // 2), max(1, warmup))
+        # at least ~40 ms of timed work per product, so that sub-millisecond
+        # calls are averaged over many launches
+        est = time_steps(fa, fn, 2, max(1, warmup))
+        reps = max(max(3, steps // 2), min(200, int(40.0 / max(est, 1e-3))))
+        ms = time_steps(fa, fn, reps, 1)
         flops = (8.0 if dt in ("c64", "c32") else 2.0) * n ** 3
         rate = flops / (ms * 1e-3) / 1e12
         peak_key = {"c64": "f64", "c32": "f32", "i16": "i8", "quant": "i8"}.get(dt, dt)
         peak = PEAK[peak_key] / (4.0 if dt == "i16" else 1.0)  # i16 = 4 int8 products
         out[name] = {"ms": round(ms, 4), "rate": round(rate, 2),
                      "unit": "TOP/s" if dt in ("i8", "i16", "quant") else "TFLOP/s",
-                     "frac_of_peak": round(rate / peak, 4)}
+                     "frac_of_peak": round(rate / peak, 4), "calls_timed": reps}
         log(f"{name}: {ms:.3f} ms  {rate:.1f}  ({rate / peak * 100:.1f}% of peak)")
         for buf in (dA, dB, dC):
             buf.free()
@@ -500,6 +504,76 @@ def serving(fa, count: int = 20000):
     return out
 
 
+def vendor_products(n: int = 8192):
+    """The same n^3 products through the vendor libraries torch calls on ROCm
+    (rocBLAS / hipBLASLt): a comparison figure only, nothing of the library
+    runs through torch.  Measured in a child process, since torch's own HIP
+    runtime does not find the device in a process where the library's
+    runtime already runs."""
+    import subprocess
+    res = subprocess.run([sys.executable, os.path.abspath(__file__), "--vendor-child", str(n)],
+                         capture_output=True, text=True, timeout=600)
+    sys.stderr.write(res.stderr)
+    if res.returncode != 0:
+        return {"error": f"child exited with {res.returncode}"}
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def vendor_child(n: int, steps: int = 10, warmup: int = 3):
+    """Times the vendor products in this (fresh) process; see vendor_products."""
+    import torch
+    torch.backends.cuda.matmul.allow_tf32 = False
+    dev = torch.device("cuda:0")
+
+    def timed(fn):
+        # the sweep's rule: at least ~40 ms of timed work per product
+        def run(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        est = run(2)
+        return run(max(steps // 2, min(200, int(40.0 / max(est, 1e-3)))))
+
+    out = {"library": "torch " + torch.__version__ + " (rocBLAS / hipBLASLt)"}
+    g = torch.Generator(device=dev).manual_seed(7)
+    work = 2.0 * n ** 3
+    for name, dt in (("f64", torch.float64), ("f32", torch.float32), ("bf16", torch.bfloat16),
+                     ("f16", torch.float16)):
+        a = torch.randn(n, n, device=dev, dtype=dt, generator=g)
+        b = torch.randn(n, n, device=dev, dtype=dt, generator=g)
+        ms = timed(lambda: torch.matmul(a, b))
+        out[name] = {"ms": round(ms, 4), "rate": round(work / (ms * 1e-3) / 1e12, 2),
+                     "unit": "TFLOP/s", "call": "torch.matmul, row-major A and B"}
+        log(f"vendor {name}: {ms:.3f} ms")
+        del a, b
+    a = torch.randint(-128, 128, (n, n), device=dev, dtype=torch.int8, generator=g)
+    b = torch.randint(-128, 128, (n, n), device=dev, dtype=torch.int8, generator=g)
+    bt = b.t().contiguous().t()  # the layout the vendor int8 kernels prefer
+    for key, bb, lay in (("i8", b, "row-major B"), ("i8_bcol", bt, "column-major B")):
+        ms = timed(lambda: torch._int_mm(a, bb))
+        out[key] = {"ms": round(ms, 4), "rate": round(work / (ms * 1e-3) / 1e12, 2),
+                    "unit": "TOP/s", "call": f"torch._int_mm, {lay}"}
+        log(f"vendor {key}: {ms:.3f} ms")
+    del a, b, bt
+    a = torch.randn(n, n, device=dev, generator=g).to(torch.float8_e4m3fn)
+    b = torch.randn(n, n, device=dev, generator=g).to(torch.float8_e4m3fn).t().contiguous().t()
+    one = torch.tensor(1.0, device=dev)
+    ms = timed(lambda: torch._scaled_mm(a, b, scale_a=one, scale_b=one, out_dtype=torch.float32))
+    out["e4m3"] = {"ms": round(ms, 4), "rate": round(work / (ms * 1e-3) / 1e12, 2),
+                   "unit": "TFLOP/s", "call": "torch._scaled_mm, column-major B, f32 out"}
+    log(f"vendor e4m3: {ms:.3f} ms")
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -516,10 +590,17 @@ def main() -> int:
     ap.add_argument("--crossover", action="store_true",
                     help="also time host-pointer products against the CPU restatement to find "
                          "the size from which offloading pays, per dtype")
+    ap.add_argument("--vendor", action="store_true",
+                    help="also time the same 8192^3 products through the vendor libraries "
+                         "(rocBLAS / hipBLASLt via torch), for comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 1 GiB device-to-device copy, the reference point "
                          "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--vendor-child", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.vendor_child:
+        print(json.dumps(vendor_child(args.vendor_child)))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the GEMM path runs on a single MI355X); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -638,6 +719,8 @@ def main() -> int:
             result["serving_p99_ms"] = b["p99_ms"]
     if args.crossover:
         result["offload_crossover"] = offload_crossover(fa)
+    if args.vendor:
+        result["vendor_products"] = vendor_products(n)
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle restatement of ferray's Goto DGEMM)")
         result["cpu_baseline"] = cpu_baseline()
