@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "mfma_nt.hpp"
 #include "real_tile.hpp"
 
 namespace fg {
@@ -325,6 +326,178 @@ void gemm_real_kernel(RealArgs<T> p) {
   }
 }
 
+// ---------------------------------------------------------------------
+// SGEMM on 256 x 256 block tiles
+// ---------------------------------------------------------------------
+// Real f32 products whose tiles all lie inside C: four waves of 128 x 128
+// (8 x 8 MFMA tiles, 256 accumulator registers, one wave per SIMD).  A stage
+// is 16 k of both panels (32 KiB), copied HBM -> LDS by global_load_lds
+// without register staging, two stages in LDS, and the NT kernel's rotated
+// pipeline: a stage's last k-step is multiplied after the barrier that opens
+// the next stage.  Per output element it issues the same MFMA k-steps in the
+// same order as gemm_real_kernel, so the results are bit-identical; four
+// times the MFMA work per fragment byte and block.
+//   A stage: 256 rows of 16 k (64 bytes); 16-byte chunk c of row r at slot
+//            c ^ ((r >> 2) & 3), so the 16 rows x 4 k of a fragment read hit
+//            64 distinct banks.
+//   B stage: 16 k-rows of 256 columns (1 KiB); 64-byte segment g of k-row r
+//            at segment g ^ (r & 3), so the 4 k-rows x 16 columns of a
+//            fragment read hit 64 distinct banks.
+constexpr int SB_M = 256, SB_N = 256, SB_K = 16;
+constexpr int SB_ABYTES = SB_M * SB_K * 4, SB_STAGE = SB_ABYTES + SB_K * SB_N * 4;
+
+__device__ __forceinline__ void sb_tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
+  const int G = 4;  // one XCD's share of tile rows at 8192
+  const int nt = tiles_m * tiles_n;
+  int bid = blockIdx.x;
+  if ((nt & 7) == 0) bid = (bid & 7) * (nt >> 3) + (bid >> 3);
+  const int per_group = G * tiles_n, group = bid / per_group, first_m = group * G;
+  const int gsz = min(tiles_m - first_m, G), in_group = bid - group * per_group;
+  tm = first_m + in_group % gsz;
+  tn = in_group / gsz;
+}
+
+template <bool ALPHA_ONE, bool BETA_ZERO>
+__global__ __launch_bounds__(256) void sgemm_big_kernel(RealArgs<float> p) {
+  using MF = Mfma16x16x4<float>;
+  constexpr int TI = 8, TJ = 8, NKK = SB_K / 4;
+  constexpr int NMF = TI * TJ, NRD = TI + TJ, NLD = 8;
+  extern __shared__ __attribute__((aligned(16))) uint8_t sb_smem[];
+  const int tiles_m = p.M / SB_M, tiles_n = p.N / SB_N;
+  int tm, tn;
+  sb_tile_coords(tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * SB_M, n0 = tn * SB_N;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm0 = (wave >> 1) * 128, wn0 = (wave & 1) * 128;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  // Copy instruction q of wave w: A rows 16 (4q + w) + lane / 4 (slot lane & 3,
+  // whose row has (r >> 2) & 3 = lane >> 4), and B k-row 4q + w (physical
+  // segment lane / 4 holds logical segment (lane / 4) ^ w).  Offsets within
+  // a stage are 32-bit (the launcher checks that 256 rows fit).
+  const float* A = p.a + (size_t)m0 * p.lda;
+  const float* B = p.b + n0;
+  const unsigned a_off = (unsigned)(((16 * wave + (lane >> 2)) * p.lda + 4 * ((lane & 3) ^ (lane >> 4))) * 4);
+  const unsigned a_q = (unsigned)(64 * p.lda * 4);
+  const unsigned b_off = (unsigned)((wave * p.ldb + 16 * ((lane >> 2) ^ wave) + 4 * (lane & 3)) * 4);
+  const unsigned b_q = (unsigned)(4 * p.ldb * 4);
+  auto issue = [&](int buf, int k0) __attribute__((always_inline)) {
+    uint8_t* as = sb_smem + buf * SB_STAGE + wave * 1024;
+    uint8_t* bs = sb_smem + buf * SB_STAGE + SB_ABYTES + wave * 1024;
+    const uint8_t* ga = reinterpret_cast<const uint8_t*>(A + k0);
+    const uint8_t* gb = reinterpret_cast<const uint8_t*>(B + (size_t)k0 * p.ldb);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      __builtin_amdgcn_global_load_lds((nt_gvoid*)(ga + (a_off + q * a_q)),
+                                       (nt_lvoid*)(as + q * 4096), 16, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      __builtin_amdgcn_global_load_lds((nt_gvoid*)(gb + (b_off + q * b_q)),
+                                       (nt_lvoid*)(bs + q * 4096), 16, 0, 0);
+  };
+  // Fragments of k-step kk: A[wm0 + 16 i + lr][4 kk + lk], B[4 kk + lk][wn0 + 16 j + lr].
+  const int a_rd = (wm0 + lr) * 64 + 4 * lk, a_sw = lr >> 2;
+  const int b_rd = SB_ABYTES + lk * 1024 + 4 * lr, b_seg = wn0 >> 4;
+  auto read_step = [&](const uint8_t* s, int kk, float* fa, float* fb) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+      fa[i] = *reinterpret_cast<const float*>(s + a_rd + i * 1024 + ((kk ^ a_sw) << 4));
+#pragma unroll
+    for (int j = 0; j < TJ; ++j)
+      fb[j] = *reinterpret_cast<const float*>(s + b_rd + kk * 4096 + (((b_seg + j) ^ lk) << 6));
+  };
+
+  f4 acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  auto mma = [&](const float* fa, const float* fb) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) acc[i][j] = MF::run(fa[i], fb[j], acc[i][j]);
+  };
+
+  float pa[TI], pb[TJ];  // last k-step of the previous stage
+  auto body = [&](auto pend_c, auto more_c, int cur, int k_next) __attribute__((always_inline)) {
+    constexpr bool PEND = decltype(pend_c)::value, MORE = decltype(more_c)::value;
+    const uint8_t* s = sb_smem + cur * SB_STAGE;
+    float fa[NKK - 1][TI], fb[NKK - 1][TJ];
+    read_step(s, 0, fa[0], fb[0]);
+    if (MORE) issue(cur ^ 1, k_next);
+    if (PEND) mma(pa, pb);
+#pragma unroll
+    for (int kk = 0; kk + 1 < NKK; ++kk) {
+      if (kk + 2 < NKK) read_step(s, kk + 1, fa[kk + 1], fb[kk + 1]);
+      else read_step(s, kk + 1, pa, pb);
+      mma(fa[kk], fb[kk]);
+    }
+    // Order for the scheduler (masks: 0x010 VMEM, 0x100 DS read, 0x008
+    // MFMA), as in the NT kernel.
+    __builtin_amdgcn_sched_group_barrier(0x100, NRD, 0);
+    if (PEND) {
+#pragma unroll
+      for (int q = 0; q < NMF; ++q) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (MORE && q % (NMF / NLD) == 0) __builtin_amdgcn_sched_group_barrier(0x010, 1, 0);
+      }
+    } else if (MORE) {
+      __builtin_amdgcn_sched_group_barrier(0x010, NLD, 0);
+    }
+#pragma unroll
+    for (int kk = 0; kk + 1 < NKK; ++kk) {
+#pragma unroll
+      for (int q = 0; q < NRD; ++q) {
+        __builtin_amdgcn_sched_group_barrier(0x008, NMF / NRD, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+    }
+  };
+  auto barrier = [&]() __attribute__((always_inline)) {
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  using T_ = std::integral_constant<bool, true>;
+  using F_ = std::integral_constant<bool, false>;
+  const int nk = p.K / SB_K;
+  issue(0, 0);
+  __syncthreads();
+  if (nk > 1)
+    body(F_{}, T_{}, 0, SB_K);
+  else
+    body(F_{}, F_{}, 0, 0);
+  int kt = 1;
+  for (; kt + 1 < nk; ++kt) {
+    barrier();
+    body(T_{}, T_{}, kt & 1, (kt + 1) * SB_K);
+  }
+  if (kt < nk) {
+    barrier();
+    body(T_{}, F_{}, kt & 1, 0);
+  }
+  mma(pa, pb);
+
+  // Epilogue: acc[i][j][r] holds C[m0 + wm0 + 16 i + 4 lk + r][n0 + wn0 + 16 j + lr];
+  // every value leaves the accumulators through a VGPR copy.
+  float* crow = p.c + (size_t)(m0 + wm0 + 4 * lk) * p.ldc + n0 + wn0 + lr;
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float* dst = crow + (size_t)(16 * i + MF::row(0, r)) * p.ldc + 16 * j;
+        float v = nt_vgpr(acc[i][j][r]);
+        if (!ALPHA_ONE) v *= p.alpha_re;
+        if (!BETA_ZERO) v = fma(p.beta_re, *dst, v);
+        *dst = v;
+      }
+}
+
 // C = beta * C for k == 0 (outer.rs:214-231, :1249-1266).
 template <typename T, bool CPLX>
 __global__ void scale_c_kernel(T* c, size_t count, T beta_re, T beta_im) {
@@ -457,6 +630,30 @@ static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, siz
                        (lda * mult * sizeof(T)) % 16 == 0 && (ldb * mult * sizeof(T)) % 16 == 0;
   const bool fast = aligned && m % RB_M == 0 && (n * mult) % RB_N == 0 &&
                     (k * mult) % StageK<T>::value == 0;
+  if constexpr (!CPLX && sizeof(T) == 4) {
+    // 256 x 256 tiles when every one lies inside C, they fill the chip, and
+    // 256 rows of A (16 rows of B) span less than 4 GiB (32-bit copy offsets)
+    const size_t big = (m / SB_M) * (n / SB_N);
+    if (fast && m % SB_M == 0 && n % SB_N == 0 && k % SB_K == 0 && big >= 256 &&
+        big <= 0x7fffffff && (size_t)SB_M * lda * 4 < ((size_t)1 << 32)) {
+      static bool attr_set[4] = {false, false, false, false};  // racing first calls are benign
+      auto go = [&](void (*kern)(RealArgs<float>), int form) {
+        if (!attr_set[form]) {
+          FG_CHECK(hipFuncSetAttribute((const void*)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 2 * SB_STAGE));
+          attr_set[form] = true;
+        }
+        hipLaunchKernelGGL(kern, dim3((unsigned)big), dim3(256), 2 * SB_STAGE, st, p);
+        FG_CHECK(hipGetLastError());
+        return FERRAY_OK;
+      };
+      if (p.alpha_one)
+        return p.beta_zero ? go(sgemm_big_kernel<true, true>, 0)
+                           : go(sgemm_big_kernel<true, false>, 1);
+      return p.beta_zero ? go(sgemm_big_kernel<false, true>, 2)
+                         : go(sgemm_big_kernel<false, false>, 3);
+    }
+  }
   if (fast) {
     hipLaunchKernelGGL((gemm_real_kernel<T, CPLX, false>), dim3((unsigned)tiles), dim3(R_THREADS),
                        0, st, p);

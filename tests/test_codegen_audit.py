@@ -32,7 +32,7 @@ HIPCC = fb.HIPCC if os.path.exists(fb.HIPCC) else shutil.which("hipcc")
 pytestmark = pytest.mark.skipif(HIPCC is None, reason="hipcc not available")
 
 # Sources whose kernels use the LDS-DMA main loop (mfma_nt.hpp).
-DMA_SOURCES = ["gemm_nt.hip", "ozaki.hip", "sgemm_split.hip"]
+DMA_SOURCES = ["gemm_nt.hip", "ozaki.hip", "sgemm_split.hip", "gemm_real.hip"]
 
 AGPR_WRITE = re.compile(r"^\s*(global|buffer|flat|scratch)_(store|atomic)\w*\s.*,\s*a[\[0-9]")
 LDS_DMA = re.compile(r"^\s*(global|buffer)_load_lds_\w+")
