@@ -632,10 +632,12 @@ static int launch_real(size_t m, size_t n, size_t k, T ar, T ai, const T* a, siz
                     (k * mult) % StageK<T>::value == 0;
   if constexpr (!CPLX && sizeof(T) == 4) {
     // 256 x 256 tiles when every one lies inside C, they fill the chip, and
-    // 256 rows of A (16 rows of B) span less than 4 GiB (32-bit copy offsets)
+    // 256 rows of A and 16 rows of B each span less than 2 GiB (the copy
+    // offsets within a stage are 32-bit)
     const size_t big = (m / SB_M) * (n / SB_N);
     if (fast && m % SB_M == 0 && n % SB_N == 0 && k % SB_K == 0 && big >= 256 &&
-        big <= 0x7fffffff && (size_t)SB_M * lda * 4 < ((size_t)1 << 32)) {
+        big <= 0x7fffffff && (size_t)SB_M * lda * 4 < ((size_t)1 << 31) &&
+        (size_t)SB_K * ldb * 4 < ((size_t)1 << 31)) {
       static bool attr_set[4] = {false, false, false, false};  // racing first calls are benign
       auto go = [&](void (*kern)(RealArgs<float>), int form) {
         if (!attr_set[form]) {
