@@ -556,11 +556,16 @@ __global__ __launch_bounds__(256) void mfma_i16_dma_kernel(I16Operands p) {
 // i16 rows -> hi byte plane (a >> 8) and lo' plane ((a & 255) - 128), k
 // padded with zeros to Kp; one block per row of the Mp-row planes (rows past
 // M are zero).  With `rowsum`, also the wrapping sum of each row's values.
+// `zero` holds gridDim.x * zper words, all cleared.
 __global__ __launch_bounds__(256) void split_i16_rows_kernel(const int16_t* __restrict__ a, int M,
                                                              int K, int Kp, uint8_t* hi,
-                                                             uint8_t* lo, int32_t* rowsum) {
+                                                             uint8_t* lo, int32_t* rowsum,
+                                                             uint32_t* zero, int zper) {
   __shared__ uint32_t part[256];
   const int row = blockIdx.x;
+  // Side job: block r clears words r*zper .. +zper-1 of `zero` (the column
+  // sums that the B pass accumulates next), which saves a launch.
+  for (int t = threadIdx.x; t < zper; t += blockDim.x) zero[(size_t)row * zper + t] = 0;
   const int16_t* src = a + (size_t)row * K;
   uint8_t* dh = hi + (size_t)row * Kp;
   uint8_t* dl = lo + (size_t)row * Kp;
@@ -759,7 +764,9 @@ static int run_i16(size_t m, size_t n, size_t k, const int16_t* a, const int16_t
   Carver cv;
   const size_t o_a0 = cv.take(Mp * Kp), o_a1 = cv.take(Mp * Kp);
   const size_t o_b0 = cv.take(Np * Kp), o_b1 = cv.take(Np * Kp);
-  const size_t o_cs = cv.take(Np * 4), o_rs = cv.take(Mp * 4);
+  // the column sums get Mp * zper words, cleared by A's split (one block per row)
+  const size_t zper = cdiv(Np, Mp);
+  const size_t o_cs = cv.take(Mp * zper * 4), o_rs = cv.take(Mp * 4);
   void* wsv = nullptr;
   if (ext) {
     if (cv.off > ext->bytes) return FERRAY_ERR_SCRATCH;
@@ -771,13 +778,13 @@ static int run_i16(size_t m, size_t n, size_t k, const int16_t* a, const int16_t
   uint8_t* ws = reinterpret_cast<uint8_t*>(wsv);
   int32_t* csp = reinterpret_cast<int32_t*>(ws + o_cs);
   int32_t* rsp = reinterpret_cast<int32_t*>(ws + o_rs);
-  hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(Np)), dim3(256), 0, st,
-                     reinterpret_cast<uint32_t*>(csp), Np);
+  // A's split also clears the column sums the B pass then accumulates
+  hipLaunchKernelGGL(split_i16_rows_kernel, dim3((unsigned)Mp), dim3(256), 0, st, a, (int)m,
+                     (int)k, (int)Kp, ws + o_a0, ws + o_a1, rsp,
+                     reinterpret_cast<uint32_t*>(csp), (int)zper);
   hipLaunchKernelGGL((pack_bt_kernel<int16_t, 1>), dim3((unsigned)(Np / 64), (unsigned)(Kp / 64)),
                      dim3(256), 0, st, b, (int)k, (int)n, (int)Kp, (int)Np, ws + o_b0, ws + o_b1,
                      csp);
-  hipLaunchKernelGGL(split_i16_rows_kernel, dim3((unsigned)Mp), dim3(256), 0, st, a, (int)m,
-                     (int)k, (int)Kp, ws + o_a0, ws + o_a1, rsp);
   FG_CHECK(hipGetLastError());
 
   constexpr int lds = 2 * 512 * 128;

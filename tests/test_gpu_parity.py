@@ -339,3 +339,15 @@ def test_half_to_f32(gpu, shape, fmt):
         if k <= 70:   # the reference's own bar at its test sizes (integer_tests.rs:458-466)
             rel = d.ravel() / np.maximum(np.abs(ref), 1.0)
             assert rel.max() < 1e-5, rel.max()
+
+
+def test_i16_wide_and_flat_shapes(gpu):
+    """Shapes whose padded column count is many times the padded row count
+    (the column sums are cleared by the row pass, several words per row)."""
+    for m, n, k in ((1, 1500, 70), (3, 40000, 33), (130, 300, 129)):
+        r = rng(m * 7 + n)
+        a = _ints(r, -32768, 32767, m * k, np.int16); b = _ints(r, -32768, 32767, k * n, np.int16)
+        c = np.zeros(m * n, np.int32); ref = np.zeros(m * n, np.int32)
+        assert gpu.gemm_i16(m, n, k, a, b, c, False)
+        oracle.gemm_i16(m, n, k, a, b, ref, False)
+        assert np.array_equal(c, ref), (m, n, k)
