@@ -311,10 +311,11 @@ __global__ __launch_bounds__(S::THREADS) void mfma_nt_kernel(NtOperands p, Epi e
 // never waits for the barrier or the first LDS reads of a stage.
 //
 // i8 uses the 16x16x64 MFMA: measured 2-3 % faster here than 32x32x32, and
-// integer results are exact either way.  bf16 / f16 / fp8 / bf8 use the NT
-// kernel's 32 x 32 forms.  The 16x16x32 bf16 / f16 form ran 7 % faster but
-// spilled accumulators to scratch; every kernel here keeps its accumulators
-// in registers and stores results from VGPRs (nt_vgpr).
+// integer results are exact either way.  bf16 / f16 use the 16x16x32 MFMA:
+// 9 % faster than 32x32x16 at 8192^3 and identical results there (each
+// sums its bf16 products into the f32 accumulator the same way).  fp8 /
+// bf8 use the NT kernel's 32 x 32 form.  Every kernel here keeps its
+// accumulators in registers and stores results from VGPRs (nt_vgpr).
 template <int ELEM>
 struct NtDmaAcc {  // 32 x 32 tiles: the NT kernel's instruction
   using A = NtAcc<ELEM>;
@@ -330,6 +331,26 @@ struct NtDmaAcc<ELEM_I8> {
   static constexpr int TS = 16, KSTEP = 64, READS = 1;
   static __device__ __forceinline__ type step(i4 a, i4 b, type c) {
     return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct NtDmaAcc<ELEM_BF16> {
+  using type = f4;
+  using frag = i4;
+  static constexpr int TS = 16, KSTEP = 64, READS = 1;
+  static __device__ __forceinline__ type step(i4 a, i4 b, type c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a),
+                                                   __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+  }
+};
+template <>
+struct NtDmaAcc<ELEM_F16> {
+  using type = f4;
+  using frag = i4;
+  static constexpr int TS = 16, KSTEP = 64, READS = 1;
+  static __device__ __forceinline__ type step(i4 a, i4 b, type c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a),
+                                                  __builtin_bit_cast(h8, b), c, 0, 0, 0);
   }
 };
 
@@ -487,23 +508,26 @@ __global__ __launch_bounds__(256) void mfma_nt_dma_kernel(NtOperands p, Epi epi)
   using F = std::integral_constant<bool, false>;
 
   const int nk = p.kbytes / KB;
+  // The last k-step is multiplied on each path separately: with the two
+  // paths joined before it, the register allocator shuffled and spilled
+  // accumulators there (the 16 x 16 bf16 / f16 forms).
   auto kloop = [&]() __attribute__((always_inline)) {
     issue(0, 0);
     __syncthreads();
-    if (nk > 1)
+    if (nk > 1) {
       body(F{}, T{}, 0, KB);
-    else
-      body(F{}, F{}, 0, 0);
-    int kt = 1;
-    for (; kt + 1 < nk; ++kt) {
-      barrier();
-      body(T{}, T{}, kt & 1, (kt + 1) * KB);
-    }
-    if (kt < nk) {
+      int kt = 1;
+      for (; kt + 1 < nk; ++kt) {
+        barrier();
+        body(T{}, T{}, kt & 1, (kt + 1) * KB);
+      }
       barrier();
       body(T{}, F{}, kt & 1, 0);
+      mma(pa, pb);
+    } else {
+      body(F{}, F{}, 0, 0);
+      mma(pa, pb);
     }
-    mma(pa, pb);
   };
   kloop();
 

@@ -101,15 +101,15 @@ __global__ __launch_bounds__(256) void split_bt_kernel(const float* __restrict__
   }
 }
 
-template <bool ALPHA_ONE, bool BETA_ZERO>
-struct EpiSgemmFast {  // EpiSgemm inside C, flags fixed
+struct EpiSgemmFast {  // EpiSgemm inside C: no bounds checks
   float* c;
   int N;
   float alpha, beta;
+  int alpha_one, beta_zero;
   __device__ __forceinline__ void operator()(int row, int col, int, float acc) const {
     float* dst = c + (size_t)row * N + col;
-    float v = ALPHA_ONE ? acc : alpha * acc;
-    if (!BETA_ZERO) v = fmaf(beta, *dst, v);
+    float v = alpha_one ? acc : alpha * acc;
+    if (!beta_zero) v = fmaf(beta, *dst, v);
     *dst = v;
   }
 };
@@ -129,15 +129,11 @@ struct EpiSgemm {
   __device__ __forceinline__ bool interior(int r0, int c0, int rows, int cols) const {
     return r0 + rows <= M && c0 + cols <= N;
   }
+  // One unchecked form (flags at run time): four fixed-flag copies of the
+  // epilogue made the kernel spill.
   template <class F>
   __device__ __forceinline__ void with_fast(F&& f) const {
-    if (alpha_one) {
-      if (beta_zero) f(EpiSgemmFast<true, true>{c, N, alpha, beta});
-      else f(EpiSgemmFast<true, false>{c, N, alpha, beta});
-    } else {
-      if (beta_zero) f(EpiSgemmFast<false, true>{c, N, alpha, beta});
-      else f(EpiSgemmFast<false, false>{c, N, alpha, beta});
-    }
+    f(EpiSgemmFast{c, N, alpha, beta, alpha_one, beta_zero});
   }
 };
 
