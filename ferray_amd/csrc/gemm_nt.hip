@@ -239,6 +239,76 @@ __global__ __launch_bounds__(256) void pack_bt_kernel(const E* __restrict__ b, i
   }
 }
 
+// B^T of row-major B (K x N) for tile-aligned shapes: ESZ-byte elements
+// (1: any 8-bit type, 2: bf16 / f16), K a multiple of 128, N of 128 / ESZ,
+// rows 16-byte aligned, no padding.  A block moves a 128 (k) x 128-byte (n)
+// tile: 16-byte loads of 128-byte row pieces into LDS (row pitch 33 words),
+// then each thread reads one word from each of 16 k rows (4 / ESZ columns)
+// and transposes them in registers into 16 k of each of its columns, stored
+// as 16-byte pieces that fill whole 128-byte runs of the output rows.
+// colsum (bytes only, optional) accumulates the wrapping int8 column sums.
+template <int ESZ>
+__global__ __launch_bounds__(256) void pack_bt_wide_kernel(const uint8_t* __restrict__ b,
+                                                           int row_bytes, int ldo_bytes,
+                                                           uint8_t* __restrict__ out,
+                                                           int32_t* colsum) {
+  constexpr int P = 33;              // LDS row pitch in words
+  constexpr int CPW = 4 / ESZ;       // columns per word
+  __shared__ uint32_t tile[128 * P];
+  const int t = threadIdx.x;
+  const size_t nb0 = (size_t)blockIdx.x * 128;  // byte column of the tile in B
+  const int k0 = blockIdx.y * 128;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int r = p * 32 + (t >> 3), c = t & 7;
+    const u4v v = *reinterpret_cast<const u4v*>(b + (size_t)(k0 + r) * row_bytes + nb0 + 16 * c);
+    uint32_t* d = tile + r * P + 4 * c;
+    d[0] = v[0];
+    d[1] = v[1];
+    d[2] = v[2];
+    d[3] = v[3];
+  }
+  __syncthreads();
+  const int g = t >> 3, c = t & 7;   // word column g, k rows 16c .. 16c + 15
+  uint32_t w[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) w[i] = tile[(16 * c + i) * P + g];
+  const size_t col0 = nb0 / ESZ + (size_t)CPW * g;  // first output row (column of B)
+  uint8_t* dst = out + col0 * ldo_bytes + (size_t)k0 * ESZ + 16 * ESZ * c;
+#pragma unroll
+  for (int j = 0; j < CPW; ++j) {
+    uint32_t o[4 * ESZ];
+    if constexpr (ESZ == 1) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        o[q] = ((w[4 * q] >> (8 * j)) & 0xffu) | (((w[4 * q + 1] >> (8 * j)) & 0xffu) << 8) |
+               (((w[4 * q + 2] >> (8 * j)) & 0xffu) << 16) | ((w[4 * q + 3] >> (8 * j)) << 24);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        o[q] = ((w[2 * q] >> (16 * j)) & 0xffffu) | ((w[2 * q + 1] >> (16 * j)) << 16);
+    }
+#pragma unroll
+    for (int h = 0; h < ESZ; ++h)
+      *reinterpret_cast<u4v*>(dst + (size_t)j * ldo_bytes + 16 * h) =
+          u4v{o[4 * h], o[4 * h + 1], o[4 * h + 2], o[4 * h + 3]};
+  }
+  if constexpr (ESZ == 1) {
+    if (colsum) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int s = 0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) s += (int)(int8_t)(w[i] >> (8 * j));
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        s += __shfl_xor(s, 4);
+        if (c == 0) atomicAdd(reinterpret_cast<unsigned int*>(colsum + col0 + j), (unsigned)s);
+      }
+    }
+  }
+}
+
 // Ap planes (Mp x Kp) from row-major A (M x K): MAP 0 and 2 as above; MAP 3
 // flips the sign bit of u8 (padding stays 0).
 template <typename E, int MAP>
@@ -658,15 +728,27 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
     hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(Np)), dim3(256), 0, st,
                        reinterpret_cast<uint32_t*>(csp), Np);
   }
-  const dim3 tgrid((unsigned)(Np / 64), (unsigned)cdiv(Kp, 64));
-  if (half)
-    hipLaunchKernelGGL((pack_bt_kernel<uint16_t, 2>), tgrid, dim3(256), 0, st,
-                       reinterpret_cast<const uint16_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
-                       ws + o_bt0, nullptr, nullptr);
-  else
-    hipLaunchKernelGGL((pack_bt_kernel<int8_t, 0>), tgrid, dim3(256), 0, st,
-                       reinterpret_cast<const int8_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
-                       ws + o_bt0, nullptr, csp);
+  if (Kp == k && Np == n && k % 128 == 0 && k / 128 <= 65535 && (n * es) % 128 == 0 &&
+      (uintptr_t)b % 16 == 0) {
+    const dim3 wgrid((unsigned)(n * es / 128), (unsigned)(k / 128));
+    if (half)
+      hipLaunchKernelGGL(pack_bt_wide_kernel<2>, wgrid, dim3(256), 0, st,
+                         reinterpret_cast<const uint8_t*>(b), (int)(n * es), (int)(k * es),
+                         ws + o_bt0, nullptr);
+    else
+      hipLaunchKernelGGL(pack_bt_wide_kernel<1>, wgrid, dim3(256), 0, st,
+                         reinterpret_cast<const uint8_t*>(b), (int)n, (int)k, ws + o_bt0, csp);
+  } else {
+    const dim3 tgrid((unsigned)(Np / 64), (unsigned)cdiv(Kp, 64));
+    if (half)
+      hipLaunchKernelGGL((pack_bt_kernel<uint16_t, 2>), tgrid, dim3(256), 0, st,
+                         reinterpret_cast<const uint16_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
+                         ws + o_bt0, nullptr, nullptr);
+    else
+      hipLaunchKernelGGL((pack_bt_kernel<int8_t, 0>), tgrid, dim3(256), 0, st,
+                         reinterpret_cast<const int8_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
+                         ws + o_bt0, nullptr, csp);
+  }
   FG_CHECK(hipGetLastError());
 
   if (need_rowsum && !flip_a) {
