@@ -118,6 +118,27 @@ def test_bf16_large(gpu, m, n, k):
 
 
 @pytest.mark.parametrize("m,n,k", SHAPES)
+def test_f16_large(gpu, m, n, k):
+    r = rng(307)
+    a16 = r.standard_normal(m * k).astype(np.float16)
+    b16 = r.standard_normal(k * n).astype(np.float16)
+    a, b = a16.view(np.uint16), b16.view(np.uint16)
+    A = a16.astype(np.float64).reshape(m, k)
+    B = b16.astype(np.float64).reshape(k, n)
+    for acc in (False, True):
+        c0 = r.standard_normal(m * n).astype(np.float32)
+        c = c0.copy()
+        assert gpu.gemm_f16_f32(m, n, k, a, b, c, acc)
+        ref = (A @ B).ravel() + (c0.astype(np.float64) if acc else 0.0)
+        # f16 products are exact in f32 (11-bit significands), so the bound
+        # is the one for bf16 above.
+        mag = (np.abs(A) @ np.abs(B)).ravel() + (np.abs(c0) if acc else 0.0)
+        err = np.abs(c - ref)
+        bound = (k + 2) * 2.0 ** -24 * mag
+        assert np.all(err <= bound), (int((err > bound).sum()), float(err.max()))
+
+
+@pytest.mark.parametrize("m,n,k", SHAPES)
 def test_e4m3_integer_valued_large(gpu, m, n, k):
     table = np.array([oracle.e4m3_to_f64(i) for i in range(256)])
     ok = np.flatnonzero(np.isfinite(table) & (table == np.rint(table)) & (np.abs(table) <= 8))
