@@ -235,6 +235,103 @@ int staged(const void* a, size_t abytes, const void* b, size_t bbytes, void* c, 
   return FERRAY_OK;
 }
 
+// A thread's second and third streams and a pool of events for the
+// row-panel pipeline below.
+struct PanelStreams {
+  hipStream_t comp = nullptr, down = nullptr;
+  std::vector<hipEvent_t> ev;
+  ~PanelStreams() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    if (comp) {
+      release_workspace(comp);
+      (void)hipStreamDestroy(comp);
+    }
+    if (down) (void)hipStreamDestroy(down);
+  }
+  int init(size_t events) {
+    if (!comp) FG_CHECK(hipStreamCreateWithFlags(&comp, hipStreamNonBlocking));
+    if (!down) FG_CHECK(hipStreamCreateWithFlags(&down, hipStreamNonBlocking));
+    while (ev.size() < events) {
+      hipEvent_t e;
+      FG_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      ev.push_back(e);
+    }
+    return FERRAY_OK;
+  }
+};
+thread_local PanelStreams t_panels;
+
+// Number of row panels for a product with an m x n C: one per 4 Mi elements
+// of C, at most 8, so each panel still fills the chip with tiles.
+inline size_t row_panels(size_t m, size_t n) {
+  return std::min<size_t>(8, (m * n) >> 22);
+}
+
+// Like staged(), for products whose rows of C depend only on the same rows
+// of A (and of C when it is read): B goes up first, then A (with C) in row
+// panels, and each panel's product is launched on a second stream as soon
+// as its rows have landed, so the uploads of later panels and the downloads
+// of earlier ones (a third stream) overlap the products.  The panel products
+// are the same kernels' results as one call over all rows (every element's
+// k-sequence is fixed by the kernels, not by the row count), so this is used
+// only where that holds.  Falls back to staged() below 64 MiB of traffic or
+// with fewer than two panels.
+// launch(row0, rows, dA_rows, dB, dC_rows, stream) runs one panel.
+template <typename F>
+int staged_rows(const void* a, size_t a_row, const void* b, size_t bbytes, void* c, size_t c_row,
+                size_t m, size_t n, bool c_in, F&& launch) {
+  const size_t panels = row_panels(m, n);
+  const size_t traffic = m * a_row + bbytes + m * c_row * (c_in ? 2 : 1);
+  if (panels < 2 || traffic < ((size_t)64 << 20))
+    return staged(a, m * a_row, b, bbytes, c, m * c_row, c_in,
+                  [&](void* da, void* db, void* dc, hipStream_t st) {
+                    return launch((size_t)0, m, da, db, dc, st);
+                  });
+  int s = ensure_device();
+  if (s) return s;
+  hipStream_t st = resolve_stream(nullptr);
+  if (!st) return FERRAY_ERR_HIP;
+  if ((s = t_panels.init(2 * panels))) return s;
+  hipStream_t sc = t_panels.comp, sd = t_panels.down;
+  void *da = nullptr, *db = nullptr, *dc = nullptr;
+  if ((s = t_staging.get(0, m * a_row, &da))) return s;
+  if ((s = t_staging.get(1, bbytes, &db))) return s;
+  if ((s = t_staging.get(2, m * c_row, &dc))) return s;
+  auto fail = [&](int e) {
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamSynchronize(sc);
+    (void)hipStreamSynchronize(sd);
+    return e;
+  };
+  const size_t rows = cdiv(cdiv(m, panels), 128) * 128;
+  auto A = [&](size_t r) { return (char*)da + r * a_row; };
+  auto C = [&](size_t r) { return (char*)dc + r * c_row; };
+  if (bbytes && (s = host_upload(db, b, bbytes, st))) return fail(s);
+  size_t np = 0;
+  for (size_t r0 = 0; r0 < m; r0 += rows, ++np) {
+    const size_t rr = std::min(rows, m - r0);
+    hipEvent_t up = t_panels.ev[2 * np], done = t_panels.ev[2 * np + 1];
+    if ((s = host_upload(A(r0), (const char*)a + r0 * a_row, rr * a_row, st))) return fail(s);
+    if (c_in && (s = host_upload(C(r0), (const char*)c + r0 * c_row, rr * c_row, st)))
+      return fail(s);
+    if (hipEventRecord(up, st) != hipSuccess || hipStreamWaitEvent(sc, up, 0) != hipSuccess)
+      return fail(FERRAY_ERR_HIP);
+    if ((s = launch(r0, rr, (void*)A(r0), db, (void*)C(r0), sc))) return fail(s);
+    if (hipEventRecord(done, sc) != hipSuccess) return fail(FERRAY_ERR_HIP);
+  }
+  np = 0;
+  for (size_t r0 = 0; r0 < m; r0 += rows, ++np) {
+    const size_t rr = std::min(rows, m - r0);
+    if (hipStreamWaitEvent(sd, t_panels.ev[2 * np + 1], 0) != hipSuccess)
+      return fail(FERRAY_ERR_HIP);
+    if ((s = host_download((char*)c + r0 * c_row, C(r0), rr * c_row, sd))) return fail(s);
+  }
+  if (hipStreamSynchronize(sd) != hipSuccess || hipStreamSynchronize(sc) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return FERRAY_ERR_HIP;
+  return FERRAY_OK;
+}
+
 inline bool finish(int s) {
   ferray_set_last_status_internal(s);
   return s == FERRAY_OK;
@@ -250,11 +347,13 @@ extern "C" {
 bool ferray_gemm_f64(size_t m, size_t n, size_t k, double alpha, const double* a,
                      const double* b, double beta, double* c) {
   if (m == 0 || n == 0) return finish(FERRAY_OK);
-  return finish(staged(a, m * k * 8, b, k * n * 8, c, m * n * 8, beta != 0.0,
-                       [&](void* da, void* db, void* dc, hipStream_t st) {
-                         return ferray_gemm_f64_dev(m, n, k, alpha, (const double*)da,
-                                                    (const double*)db, beta, (double*)dc, st);
-                       }));
+  return finish(staged_rows(a, k * 8, b, k * n * 8, c, n * 8, m, n, beta != 0.0,
+                            [&](size_t, size_t rows, void* da, void* db, void* dc,
+                                hipStream_t st) {
+                              return ferray_gemm_f64_dev(rows, n, k, alpha, (const double*)da,
+                                                         (const double*)db, beta, (double*)dc,
+                                                         st);
+                            }));
 }
 
 bool ferray_matmul_batched(int dtype, size_t batch, size_t a_batch, size_t b_batch, size_t m,
@@ -340,11 +439,12 @@ bool ferray_dgemm_ozaki(size_t m, size_t n, size_t k, double alpha, const double
 bool ferray_gemm_f32(size_t m, size_t n, size_t k, float alpha, const float* a, const float* b,
                      float beta, float* c) {
   if (m == 0 || n == 0) return finish(FERRAY_OK);
-  return finish(staged(a, m * k * 4, b, k * n * 4, c, m * n * 4, beta != 0.0f,
-                       [&](void* da, void* db, void* dc, hipStream_t st) {
-                         return ferray_gemm_f32_dev(m, n, k, alpha, (const float*)da,
-                                                    (const float*)db, beta, (float*)dc, st);
-                       }));
+  return finish(staged_rows(a, k * 4, b, k * n * 4, c, n * 4, m, n, beta != 0.0f,
+                            [&](size_t, size_t rows, void* da, void* db, void* dc,
+                                hipStream_t st) {
+                              return ferray_gemm_f32_dev(rows, n, k, alpha, (const float*)da,
+                                                         (const float*)db, beta, (float*)dc, st);
+                            }));
 }
 
 bool ferray_gemm_c64(size_t m, size_t n, size_t k, double alpha_re, double alpha_im,
@@ -352,70 +452,76 @@ bool ferray_gemm_c64(size_t m, size_t n, size_t k, double alpha_re, double alpha
                      double* c) {
   if (m == 0 || n == 0) return finish(FERRAY_OK);
   bool c_in = !(beta_re == 0.0 && beta_im == 0.0);
-  return finish(staged(a, m * k * 16, b, k * n * 16, c, m * n * 16, c_in,
-                       [&](void* da, void* db, void* dc, hipStream_t st) {
-                         return ferray_gemm_c64_dev(m, n, k, alpha_re, alpha_im,
-                                                    (const double*)da, (const double*)db,
-                                                    beta_re, beta_im, (double*)dc, st);
-                       }));
+  return finish(staged_rows(a, k * 16, b, k * n * 16, c, n * 16, m, n, c_in,
+                            [&](size_t, size_t rows, void* da, void* db, void* dc,
+                                hipStream_t st) {
+                              return ferray_gemm_c64_dev(rows, n, k, alpha_re, alpha_im,
+                                                         (const double*)da, (const double*)db,
+                                                         beta_re, beta_im, (double*)dc, st);
+                            }));
 }
 
 bool ferray_gemm_c32(size_t m, size_t n, size_t k, float alpha_re, float alpha_im,
                      const float* a, const float* b, float beta_re, float beta_im, float* c) {
   if (m == 0 || n == 0) return finish(FERRAY_OK);
   bool c_in = !(beta_re == 0.0f && beta_im == 0.0f);
-  return finish(staged(a, m * k * 8, b, k * n * 8, c, m * n * 8, c_in,
-                       [&](void* da, void* db, void* dc, hipStream_t st) {
-                         return ferray_gemm_c32_dev(m, n, k, alpha_re, alpha_im,
-                                                    (const float*)da, (const float*)db, beta_re,
-                                                    beta_im, (float*)dc, st);
-                       }));
+  return finish(staged_rows(a, k * 8, b, k * n * 8, c, n * 8, m, n, c_in,
+                            [&](size_t, size_t rows, void* da, void* db, void* dc,
+                                hipStream_t st) {
+                              return ferray_gemm_c32_dev(rows, n, k, alpha_re, alpha_im,
+                                                         (const float*)da, (const float*)db,
+                                                         beta_re, beta_im, (float*)dc, st);
+                            }));
 }
 
 bool ferray_gemm_i8(size_t m, size_t n, size_t k, const uint8_t* a, const int8_t* b, int32_t* c,
                     bool accumulate) {
   if (m == 0 || n == 0) return finish(FERRAY_OK);
-  return finish(staged(a, m * k, b, k * n, c, m * n * 4, accumulate,
-                       [&](void* da, void* db, void* dc, hipStream_t st) {
-                         return ferray_gemm_i8_dev(m, n, k, (const uint8_t*)da,
-                                                   (const int8_t*)db, (int32_t*)dc, accumulate,
-                                                   st);
-                       }));
+  return finish(staged_rows(a, k, b, k * n, c, n * 4, m, n, accumulate,
+                            [&](size_t, size_t rows, void* da, void* db, void* dc,
+                                hipStream_t st) {
+                              return ferray_gemm_i8_dev(rows, n, k, (const uint8_t*)da,
+                                                        (const int8_t*)db, (int32_t*)dc,
+                                                        accumulate, st);
+                            }));
 }
 
 bool ferray_gemm_i8_signed(size_t m, size_t n, size_t k, const int8_t* a, const int8_t* b,
                            int32_t* c, bool accumulate) {
   if (m == 0 || n == 0) return finish(FERRAY_OK);
-  return finish(staged(a, m * k, b, k * n, c, m * n * 4, accumulate,
-                       [&](void* da, void* db, void* dc, hipStream_t st) {
-                         return ferray_gemm_i8_signed_dev(m, n, k, (const int8_t*)da,
-                                                          (const int8_t*)db, (int32_t*)dc,
-                                                          accumulate, st);
-                       }));
+  return finish(staged_rows(a, k, b, k * n, c, n * 4, m, n, accumulate,
+                            [&](size_t, size_t rows, void* da, void* db, void* dc,
+                                hipStream_t st) {
+                              return ferray_gemm_i8_signed_dev(rows, n, k, (const int8_t*)da,
+                                                               (const int8_t*)db, (int32_t*)dc,
+                                                               accumulate, st);
+                            }));
 }
 
 bool ferray_gemm_i16(size_t m, size_t n, size_t k, const int16_t* a, const int16_t* b,
                      int32_t* c, bool accumulate) {
   if (m == 0 || n == 0) return finish(FERRAY_OK);
-  return finish(staged(a, m * k * 2, b, k * n * 2, c, m * n * 4, accumulate,
-                       [&](void* da, void* db, void* dc, hipStream_t st) {
-                         return ferray_gemm_i16_dev(m, n, k, (const int16_t*)da,
-                                                    (const int16_t*)db, (int32_t*)dc,
-                                                    accumulate, st);
-                       }));
+  return finish(staged_rows(a, k * 2, b, k * n * 2, c, n * 4, m, n, accumulate,
+                            [&](size_t, size_t rows, void* da, void* db, void* dc,
+                                hipStream_t st) {
+                              return ferray_gemm_i16_dev(rows, n, k, (const int16_t*)da,
+                                                         (const int16_t*)db, (int32_t*)dc,
+                                                         accumulate, st);
+                            }));
 }
 
 bool ferray_quantized_matmul(size_t m, size_t n, size_t k, const uint8_t* a, uint8_t a_zp,
                              float scale_a, const int8_t* b, int8_t b_zp, float scale_b,
                              float* c, bool accumulate) {
   if (m == 0 || n == 0) return finish(FERRAY_OK);
-  return finish(staged(a, m * k, b, k * n, c, m * n * 4, accumulate,
-                       [&](void* da, void* db, void* dc, hipStream_t st) {
-                         return ferray_quantized_matmul_dev(m, n, k, (const uint8_t*)da, a_zp,
-                                                            scale_a, (const int8_t*)db, b_zp,
-                                                            scale_b, (float*)dc, accumulate,
-                                                            st);
-                       }));
+  return finish(staged_rows(a, k, b, k * n, c, n * 4, m, n, accumulate,
+                            [&](size_t, size_t rows, void* da, void* db, void* dc,
+                                hipStream_t st) {
+                              return ferray_quantized_matmul_dev(rows, n, k, (const uint8_t*)da,
+                                                                 a_zp, scale_a, (const int8_t*)db,
+                                                                 b_zp, scale_b, (float*)dc,
+                                                                 accumulate, st);
+                            }));
 }
 
 bool ferray_gemm_bf16_f32(size_t m, size_t n, size_t k, const uint16_t* a, const uint16_t* b,
