@@ -57,4 +57,15 @@ int host_download(void* dst, const void* src, size_t bytes, hipStream_t stream);
 
 inline size_t cdiv(size_t a, size_t b) { return (a + b - 1) / b; }
 
+// A grid's y extent is at most 65535 blocks.  Kernels whose grid y walks the
+// k dimension (k / 64 tiles reach it at k = 4 Mi) take the first y index of
+// their launch as an argument y0 and add it to blockIdx.y; for_grid_y calls
+// launch(y0, ny) for consecutive slices of at most 65535 covering [0, total).
+constexpr size_t kMaxGridY = 65535;
+template <class F>
+inline void for_grid_y(size_t total, F&& launch) {
+  for (size_t y0 = 0; y0 < total; y0 += kMaxGridY)
+    launch((int)y0, (unsigned)(total - y0 < kMaxGridY ? total - y0 : kMaxGridY));
+}
+
 }  // namespace fg

@@ -174,17 +174,17 @@ struct EpiF32 {
 // ---------------------------------------------------------------------
 
 // Bt planes from row-major B (K x N) of element E, tile 64 (k) x 64 (n),
-// zero padded to Np x Kp:
+// grid (Np / 64, Kp / 64) launched in y slices from y0, zero padded to Np x Kp:
 //   MAP 0: byte copy      MAP 1: i16 -> hi byte plane + (lo - 128) plane
 //   MAP 2: 16-bit copy
 // colsum (optional) accumulates the wrapping column sums of B.
 template <typename E, int MAP>
 __global__ __launch_bounds__(256) void pack_bt_kernel(const E* __restrict__ b, int K, int N,
                                                       int Kp, int Np, uint8_t* out0,
-                                                      uint8_t* out1, int32_t* colsum) {
+                                                      uint8_t* out1, int32_t* colsum, int y0) {
   __shared__ E tile[64][64 + 4];
   const int tid = threadIdx.x;
-  const int n0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  const int n0 = blockIdx.x * 64, k0 = (y0 + (int)blockIdx.y) * 64;
   {
     const int row = tid >> 2, c0 = (tid & 3) * 16;
     const int k = k0 + row;
@@ -739,15 +739,17 @@ static int run_nt_shape(int kind, size_t m, size_t n, size_t k, const void* a, c
       hipLaunchKernelGGL(pack_bt_wide_kernel<1>, wgrid, dim3(256), 0, st,
                          reinterpret_cast<const uint8_t*>(b), (int)n, (int)k, ws + o_bt0, csp);
   } else {
-    const dim3 tgrid((unsigned)(Np / 64), (unsigned)cdiv(Kp, 64));
-    if (half)
-      hipLaunchKernelGGL((pack_bt_kernel<uint16_t, 2>), tgrid, dim3(256), 0, st,
-                         reinterpret_cast<const uint16_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
-                         ws + o_bt0, nullptr, nullptr);
-    else
-      hipLaunchKernelGGL((pack_bt_kernel<int8_t, 0>), tgrid, dim3(256), 0, st,
-                         reinterpret_cast<const int8_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
-                         ws + o_bt0, nullptr, csp);
+    for_grid_y(cdiv(Kp, 64), [&](int y0, unsigned ny) {
+      const dim3 tgrid((unsigned)(Np / 64), ny);
+      if (half)
+        hipLaunchKernelGGL((pack_bt_kernel<uint16_t, 2>), tgrid, dim3(256), 0, st,
+                           reinterpret_cast<const uint16_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
+                           ws + o_bt0, nullptr, nullptr, y0);
+      else
+        hipLaunchKernelGGL((pack_bt_kernel<int8_t, 0>), tgrid, dim3(256), 0, st,
+                           reinterpret_cast<const int8_t*>(b), (int)k, (int)n, (int)Kp, (int)Np,
+                           ws + o_bt0, nullptr, csp, y0);
+    });
   }
   FG_CHECK(hipGetLastError());
 
@@ -864,9 +866,10 @@ static int run_i16(size_t m, size_t n, size_t k, const int16_t* a, const int16_t
   hipLaunchKernelGGL(split_i16_rows_kernel, dim3((unsigned)Mp), dim3(256), 0, st, a, (int)m,
                      (int)k, (int)Kp, ws + o_a0, ws + o_a1, rsp,
                      reinterpret_cast<uint32_t*>(csp), (int)zper);
-  hipLaunchKernelGGL((pack_bt_kernel<int16_t, 1>), dim3((unsigned)(Np / 64), (unsigned)(Kp / 64)),
-                     dim3(256), 0, st, b, (int)k, (int)n, (int)Kp, (int)Np, ws + o_b0, ws + o_b1,
-                     csp);
+  for_grid_y(Kp / 64, [&](int y0, unsigned ny) {
+    hipLaunchKernelGGL((pack_bt_kernel<int16_t, 1>), dim3((unsigned)(Np / 64), ny), dim3(256), 0,
+                       st, b, (int)k, (int)n, (int)Kp, (int)Np, ws + o_b0, ws + o_b1, csp, y0);
+  });
   FG_CHECK(hipGetLastError());
 
   constexpr int lds = 2 * 512 * 128;

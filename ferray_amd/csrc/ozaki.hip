@@ -12,13 +12,15 @@
 //      <= 256) the symmetric residues of A' and B' fit int8.
 //   3. N INT8 GEMMs on the INT8 matrix cores (exact: |sum| <= k*2^14 < 2^31
 //      for k <= 2^16 per chunk), each reduced mod m_l in the epilogue and
-//      stored as one byte.
+//      stored as one byte; for longer k each further chunk adds its product
+//      to the stored residue mod m_l.
 //   4. Reconstruction (FP64).  Chinese remaindering with the weights
 //      W_l = (M/m_l) * ((M/m_l)^-1 mod m_l) split into 32-bit pieces, so every
 //      partial sum is an exact double; the multiple of M is removed exactly,
 //      the result is evaluated from balanced 32-bit digits, and scaled back by
 //      2^-(s_i + t_j) before alpha/beta.
 // Rows or columns holding Inf/NaN produce NaN in the affected outputs.
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -30,6 +32,9 @@
 namespace fg {
 
 constexpr int OZ_MAX_MODULI = 16;
+// k bytes per residue product launch: every i32 sum of a chunk is exact
+// (|residue| <= 128, so |acc| <= 2^16 * 2^14 = 2^30).
+constexpr size_t OZ_K_CHUNK = (size_t)1 << 16;
 // Pairwise coprime moduli, largest first (256 = 2^8, 255 = 3*5*17,
 // 253 = 11*23, 247 = 13*19, 217 = 7*31, the rest prime).
 static const int kModuli[OZ_MAX_MODULI] = {256, 255, 253, 251, 247, 241, 239, 233,
@@ -179,10 +184,11 @@ __device__ __forceinline__ void oz_col_partial_block(int bx, int by, const doubl
   ps[(size_t)by * N + j] = acc.S;
   if (nonfinite) atomicOr(bad + j, 1);
 }
+// Grid (column groups, k slices), launched in y slices from y0.
 __global__ void oz_col_partial(const double* __restrict__ b, int K, int N, int kslice,
                                int* __restrict__ pe, double* __restrict__ ps,
-                               int* __restrict__ bad) {
-  oz_col_partial_block(blockIdx.x, blockIdx.y, b, K, N, kslice, pe, ps, bad);
+                               int* __restrict__ bad, int y0) {
+  oz_col_partial_block(blockIdx.x, y0 + (int)blockIdx.y, b, K, N, kslice, pe, ps, bad);
 }
 
 __global__ void oz_col_finish(int N, int slices, int g, const int* __restrict__ pe,
@@ -441,8 +447,9 @@ __global__ __launch_bounds__(256) void oz_residues_bt(const double* __restrict__
                                                       int Kp, int ldp,
                                                       const int* __restrict__ shift,
                                                       uint8_t* __restrict__ planes,
-                                                      size_t plane_bytes) {
-  oz_residues_bt_tile<NM>(blockIdx.x, blockIdx.y, b, K, N, Kp, ldp, shift, planes, plane_bytes);
+                                                      size_t plane_bytes, int y0) {
+  oz_residues_bt_tile<NM>(blockIdx.x, y0 + (int)blockIdx.y, b, K, N, Kp, ldp, shift, planes,
+                          plane_bytes);
 }
 
 // GEMM epilogue: residue of the exact i32 product modulo m_plane, as a byte.
@@ -457,17 +464,23 @@ __device__ __forceinline__ uint8_t residue_byte(int acc, int mm, double inv) {
 }
 
 // EpiResidue for a block tile inside C: one plane (the block's), no bounds
-// checks (measured 5 % off the 14-moduli product at 8192^3).
+// checks (measured 5 % off the 14-moduli product at 8192^3).  ADD: a later
+// k chunk, whose product is added to the residue the earlier chunks left
+// (|acc| < 2^30, so acc + r stays exact in i32).
+template <bool ADD>
 struct EpiResidueFast {
   uint8_t* plane;
   int ld, mm;
   double inv;
   __device__ __forceinline__ void operator()(int row, int col, int, int acc) const {
-    plane[(size_t)row * ld + col] = residue_byte(acc, mm, inv);
+    uint8_t* dst = plane + (size_t)row * ld + col;
+    if (ADD) acc += *dst;
+    *dst = residue_byte(acc, mm, inv);
   }
 };
 
-struct EpiResidue {
+template <bool ADD>
+struct EpiResidueT {
   uint8_t* r;
   size_t plane_bytes;
   int ld, M, N;
@@ -475,7 +488,9 @@ struct EpiResidue {
   double inv_m[OZ_MAX_MODULI];
   __device__ __forceinline__ void operator()(int row, int col, int z, int acc) const {
     if (row >= M || col >= N) return;
-    r[(size_t)z * plane_bytes + (size_t)row * ld + col] = residue_byte(acc, m[z], inv_m[z]);
+    uint8_t* dst = r + (size_t)z * plane_bytes + (size_t)row * ld + col;
+    if (ADD) acc += *dst;
+    *dst = residue_byte(acc, m[z], inv_m[z]);
   }
   __device__ __forceinline__ bool interior(int r0, int c0, int rows, int cols) const {
     return r0 + rows <= M && c0 + cols <= N;
@@ -485,9 +500,10 @@ struct EpiResidue {
   template <class F>
   __device__ __forceinline__ void with_fast(F&& f) const {
     const int z = blockIdx.y;
-    f(EpiResidueFast{r + (size_t)z * plane_bytes, ld, m[z], inv_m[z]});
+    f(EpiResidueFast<ADD>{r + (size_t)z * plane_bytes, ld, m[z], inv_m[z]});
   }
 };
+using EpiResidue = EpiResidueT<false>;
 
 // Reconstruction: C = alpha * 2^-(s_i+t_j) * CRT(r_1..r_N) + beta * C.  Each
 // thread finishes 4 consecutive columns of one row (residue planes have a
@@ -747,8 +763,10 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
                          ws + o_pa, a_plane, (int)Mp, b, (int)n, kslice, (int)gn, (int)n_c, pe,
                          ps, badB);
     } else {
-      hipLaunchKernelGGL(oz_col_partial, dim3(gn, (unsigned)slices), dim3(256), 0, st, b, (int)k,
-                         (int)n, kslice, pe, ps, badB);
+      for_grid_y(slices, [&](int y0, unsigned ny) {
+        hipLaunchKernelGGL(oz_col_partial, dim3(gn, ny), dim3(256), 0, st, b, (int)k, (int)n,
+                           kslice, pe, ps, badB, y0);
+      });
       hipLaunchKernelGGL((oz_scale_residues_a<NM>), dim3((unsigned)Mp), dim3(256), 0, st, a,
                          (int)m, (int)k, (int)Kp, (int)ldp, vec_a, gamma, sA, badA, ws + o_pa,
                          a_plane);
@@ -758,9 +776,10 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
                      ps, tB);
   with_moduli(nmod, [&](auto nm) {
     constexpr int NM = decltype(nm)::value;
-    hipLaunchKernelGGL((oz_residues_bt<NM>), dim3((unsigned)(Np / 32), (unsigned)(Kp / 128)),
-                       dim3(256), 0, st, b, (int)k, (int)n, (int)Kp, (int)ldp, tB, ws + o_pb,
-                       b_plane);
+    for_grid_y(Kp / 128, [&](int y0, unsigned ny) {
+      hipLaunchKernelGGL((oz_residues_bt<NM>), dim3((unsigned)(Np / 32), ny), dim3(256), 0, st, b,
+                         (int)k, (int)n, (int)Kp, (int)ldp, tB, ws + o_pb, b_plane, y0);
+    });
   });
   FG_CHECK(hipGetLastError());
 
@@ -771,10 +790,9 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
   p.b_plane = b_plane;
   p.lda = ldp;
   p.ldb = ldp;
-  p.kbytes = (int)Kp;
   p.tiles_m = (int)(Mp / S::BM);
   p.tiles_n = (int)(Np / S::BN);
-  EpiResidue e{};
+  EpiResidueT<false> e{};
   e.r = ws + o_r;
   e.plane_bytes = r_plane;
   e.ld = (int)ldr;
@@ -784,7 +802,31 @@ static int ozaki_run(size_t m, size_t n, size_t k, double alpha, const double* a
     e.m[l] = kModuli[l];
     e.inv_m[l] = 1.0 / (double)kModuli[l];
   }
-  if ((s = launch_mfma_nt<ELEM_I8, S, false>(p, nmod, e, st))) return s;
+  // k in chunks of OZ_K_CHUNK: each chunk's i32 sums stay exact (|r| <= 128
+  // per residue, so |acc| <= 2^16 * 2^14), and later chunks add their
+  // product to the residues mod m, which is the residue of the whole sum.
+  for (size_t kb0 = 0; kb0 < Kp; kb0 += OZ_K_CHUNK) {
+    NtOperands q = p;
+    q.a = p.a + kb0;
+    q.bt = p.bt + kb0;
+    q.kbytes = (int)std::min<size_t>(OZ_K_CHUNK, Kp - kb0);
+    if (kb0 == 0) {
+      s = launch_mfma_nt<ELEM_I8, S, false>(q, nmod, e, st);
+    } else {
+      EpiResidueT<true> ea{};
+      ea.r = e.r;
+      ea.plane_bytes = e.plane_bytes;
+      ea.ld = e.ld;
+      ea.M = e.M;
+      ea.N = e.N;
+      for (int l = 0; l < nmod; ++l) {
+        ea.m[l] = e.m[l];
+        ea.inv_m[l] = e.inv_m[l];
+      }
+      s = launch_mfma_nt<ELEM_I8, S, false>(q, nmod, ea, st);
+    }
+    if (s) return s;
+  }
 
   const size_t units = m * cdiv(n, 256);  // one wave per 256 columns of a row
   if (cdiv(units, 4) > 0x7fffffff) return FERRAY_ERR_INVALID;
@@ -817,8 +859,7 @@ static int ozaki_dev(size_t m, size_t n, size_t k, double alpha, const double* a
   if (!c || (k && (!a || !b))) return FERRAY_ERR_INVALID;
   if (nmod == 0) nmod = OZ_MAX_MODULI;
   if (nmod < 2 || nmod > OZ_MAX_MODULI) return FERRAY_ERR_INVALID;
-  // Each int8 product chunk must stay exact in i32: k * 128^2 < 2^31.
-  if (k > (1u << 16) || m > (1u << 30) || n > (1u << 30)) return FERRAY_ERR_INVALID;
+  if (k > (1u << 30) || m > (1u << 30) || n > (1u << 30)) return FERRAY_ERR_INVALID;
   if (k == 0) {
     if (beta == 1.0) return FERRAY_OK;
     size_t blocks = cdiv(m * n, 256);

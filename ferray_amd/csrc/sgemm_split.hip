@@ -79,13 +79,14 @@ __global__ __launch_bounds__(256) void split_a_kernel(const float* __restrict__ 
 }
 
 // B (K x N f32, row-major) -> B'^T (Np x T*Kp bf16) through a 64 x 64 LDS
-// tile; grid (Np / 64, Kp / 64).
+// tile; grid (Np / 64, Kp / 64), launched in y slices from y0.
 template <int T>
 __global__ __launch_bounds__(256) void split_bt_kernel(const float* __restrict__ b, int K, int N,
-                                                       int Kp, uint16_t* __restrict__ out) {
+                                                       int Kp, uint16_t* __restrict__ out,
+                                                       int y0) {
   __shared__ float tile[64][65];
   const int tid = threadIdx.x, c = tid & 63;
-  const int n0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  const int n0 = blockIdx.x * 64, k0 = (y0 + (int)blockIdx.y) * 64;
   for (int r = tid >> 6; r < 64; r += 4) {
     const int k = k0 + r, n = n0 + c;
     tile[r][c] = (k < K && n < N) ? b[(size_t)k * N + n] : 0.0f;
@@ -157,8 +158,10 @@ int run_split(size_t m, size_t n, size_t k, float alpha, const float* a, const f
   uint16_t* pb = reinterpret_cast<uint16_t*>(reinterpret_cast<uint8_t*>(wsv) + bytes_a);
   hipLaunchKernelGGL((split_a_kernel<T>), dim3(blocks_for(Mp * Kp)), dim3(256), 0, st, a, (int)m,
                      (int)k, (int)Mp, (int)Kp, pa);
-  hipLaunchKernelGGL((split_bt_kernel<T>), dim3((unsigned)(Np / 64), (unsigned)(Kp / 64)),
-                     dim3(256), 0, st, b, (int)k, (int)n, (int)Kp, pb);
+  for_grid_y(Kp / 64, [&](int y0, unsigned ny) {
+    hipLaunchKernelGGL((split_bt_kernel<T>), dim3((unsigned)(Np / 64), ny), dim3(256), 0, st, b,
+                       (int)k, (int)n, (int)Kp, pb, y0);
+  });
   FG_CHECK(hipGetLastError());
   NtOperands p{};
   p.a = reinterpret_cast<const uint8_t*>(pa);

@@ -231,7 +231,9 @@ int ferray_trsm_dev(int dtype, int upper, size_t m, size_t n, double alpha,
  * moduli <= 256 (INT8 residues), multiplied exactly on INT8 MFMA, and
  * recombined by Chinese remaindering in FP64.  moduli in [2, 16] (0 = 16)
  * trades speed for accuracy: each modulus adds ~4 bits to the product's
- * significand.  k <= 65536.  Rows/columns holding Inf or NaN give NaN.
+ * significand.  k up to 2^30: the INT8 products run in k chunks of 65536,
+ * each added to the residues mod the modulus (exact: same result as one
+ * pass).  Rows/columns holding Inf or NaN give NaN.
  * No counterpart in the reference (its DGEMM is gemm_f64, mod.rs:197-328);
  * same argument order and meaning as ferray_gemm_f64 plus `moduli`. */
 bool ferray_dgemm_ozaki(size_t m, size_t n, size_t k, double alpha,

@@ -263,3 +263,101 @@ def test_ozaki_a_past_4gib(gpu):
     cb = np.linalg.norm(b.reshape(k, n), axis=0)[None, :]
     lim = 2.0 ** (6 - g) * ra * cb + 4 * U64 * np.abs(hi)
     assert np.all(np.abs(d) <= lim), float(np.max(np.abs(d) / lim))
+
+
+# ------------------------------------------------------------------ long k
+#
+# A grid's y extent is at most 65535 blocks; the passes whose grid y walks k
+# (64- or 128-deep tiles) are launched in slices past that.  k just above
+# 2^22 (and 2^23 for Ozaki's 128-deep tiles), ragged, on thin products.
+
+KLONG = (1 << 22) + 1000
+
+
+def _upload(arr):
+    return fa.DeviceBuffer.from_array(np.ascontiguousarray(arr))
+
+
+def _ones(shape, seed):
+    """Entries in {-1, 0, 1}: every product and partial sum below 2^24 is
+    exact in f32, so float results must equal the integer product."""
+    return rng(seed).integers(-1, 2, shape).astype(np.int64)
+
+
+@pytest.mark.parametrize("kind", ["i8", "i16"])
+def test_integer_long_k(gpu, kind):
+    m, n, k = 8, 24, KLONG
+    dt, lo, hi, fill = ((np.int8, -128, 127, fa.FILL_I8) if kind == "i8"
+                        else (np.int16, -32768, 32767, fa.FILL_I16))
+    item = np.dtype(dt).itemsize
+    dA = _filled(m * k, fill, 101, item, lo, hi)
+    dB = _filled(k * n, fill, 102, item, lo, hi)
+    dC = fa.DeviceBuffer(m * n * 4)
+    run = lib().ferray_gemm_i8_signed_dev if kind == "i8" else lib().ferray_gemm_i16_dev
+    _ok(run(m, n, k, dA.ptr, dB.ptr, dC.ptr, False, None), kind)
+    fa.sync()
+    a = dA.download(dt, m * k).reshape(m, k)
+    b = dB.download(dt, k * n).reshape(k, n)
+    c = dC.download(np.int32, m * n).reshape(m, n)
+    for d in (dA, dB, dC):
+        d.free()
+    # |sum| <= k 2^30 < 2^53: the float64 product is exact
+    assert np.array_equal(c, wrap_i32(_exact(a, b)))
+
+
+@pytest.mark.parametrize("kind", ["bf16", "e4m3", "bf16x3"])
+def test_float_long_k(gpu, kind):
+    m, n, k = 8, 24, KLONG
+    a, b = _ones((m, k), 103), _ones((k, n), 104)
+    want = (a.astype(np.float64) @ b.astype(np.float64)).astype(np.float32)
+    if kind == "bf16":   # -1, 0, 1 as bf16 bit patterns
+        code = {-1: 0xBF80, 0: 0, 1: 0x3F80}
+        enc = lambda x: np.vectorize(code.get, otypes=[np.uint16])(x)
+        run = lambda da, db, dc: lib().ferray_gemm_bf16_f32_dev(m, n, k, da, db, dc, False, None)
+    elif kind == "e4m3":  # OCP E4M3: 1.0 = 0x38
+        code = {-1: 0xB8, 0: 0, 1: 0x38}
+        enc = lambda x: np.vectorize(code.get, otypes=[np.uint8])(x)
+        run = lambda da, db, dc: lib().ferray_gemm_e4m3_f32_dev(m, n, k, da, db, dc, False, None)
+    else:
+        enc = lambda x: x.astype(np.float32)
+        run = lambda da, db, dc: lib().ferray_sgemm_bf16x3_dev(m, n, k, 1.0, da, db, 0.0, dc, 3,
+                                                               None)
+    dA, dB = _upload(enc(a)), _upload(enc(b))
+    dC = fa.DeviceBuffer(m * n * 4)
+    _ok(run(dA.ptr, dB.ptr, dC.ptr), kind)
+    fa.sync()
+    c = dC.download(np.float32, m * n).reshape(m, n)
+    for d in (dA, dB, dC):
+        d.free()
+    assert np.array_equal(c, want)
+
+
+@pytest.mark.parametrize("m,n,k", [
+    (8, 16, (1 << 23) + 1000),           # B's residue pass: Kp / 128 > 65535 tiles
+    (512, 512, 70000),                   # two k chunks on the 256 x 256 kernel
+])
+def test_ozaki_long_k(gpu, m, n, k):
+    """k past one chunk of 2^16: later chunks add their products to the
+    stored residues mod m."""
+    nm = 14
+    dA = _filled(m * k, fa.FILL_F64_NORMAL, 105, 8)
+    dB = _filled(k * n, fa.FILL_F64_NORMAL, 106, 8)
+    dC = fa.DeviceBuffer(m * n * 8)
+    _ok(lib().ferray_dgemm_ozaki_dev(m, n, k, 1.0, dA.ptr, dB.ptr, 0.0, dC.ptr, nm, None),
+        "ozaki")
+    fa.sync()
+    rows = np.arange(m) if m <= 16 else np.sort(rng(11).choice(m, 16, replace=False))
+    a = np.ascontiguousarray(_rows(dA, np.float64, rows, k))
+    b = dB.download(np.float64, k * n)
+    c = _rows(dC, np.float64, rows, n)
+    for d in (dA, dB, dC):
+        d.free()
+    hi, lo = oracle.dd_gemm_f64_sampled(len(rows), n, k, a.ravel(), b, np.arange(len(rows)),
+                                        np.arange(n))
+    d = (c - hi) - lo
+    M = math.prod(OZ_MODULI[:nm])
+    g = (M.bit_length() - 1 - 2) // 2
+    ra = np.linalg.norm(a, axis=1)[:, None]
+    cb = np.linalg.norm(b.reshape(k, n), axis=0)[None, :]
+    lim = 2.0 ** (6 - g) * ra * cb + 4 * U64 * np.abs(hi)
+    assert np.all(np.abs(d) <= lim), float(np.max(np.abs(d) / lim))
