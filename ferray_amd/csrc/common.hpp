@@ -47,6 +47,12 @@ void release_workspace(hipStream_t stream);
 // Runtime initialisation on the calling thread (device selection).
 int ensure_device();
 
+// True on the process's main thread.  Thread-local resources make no HIP
+// calls when the main thread's copies are destroyed: that happens while the
+// process tears down, when the runtime (or a tool layered on it) may already
+// be gone, so they are left to the process's end.
+bool on_main_thread();
+
 // Pageable host memory <-> device, enqueued on `stream`.  Copies of 8 MiB
 // and more go through the calling thread's pinned bounce buffers; the upload
 // returns once the host buffer may be reused, the download once the host

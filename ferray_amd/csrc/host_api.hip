@@ -29,6 +29,7 @@ struct Staging {
   void* buf[3] = {nullptr, nullptr, nullptr};
   size_t cap[3] = {0, 0, 0};
   ~Staging() {
+    if (on_main_thread()) return;  // left to the process's end
     for (int i = 0; i < 3; ++i)
       if (buf[i]) (void)hipFree(buf[i]);
   }
@@ -134,6 +135,7 @@ struct PinnedRing {
   unsigned next = 0;
   bool ready = false;
   ~PinnedRing() {
+    if (on_main_thread()) return;  // left to the process's end
     for (int i = 0; i < kSlots; ++i) {
       if (ev[i]) (void)hipEventDestroy(ev[i]);
       if (slot[i]) (void)hipHostFree(slot[i]);
@@ -241,6 +243,7 @@ struct PanelStreams {
   hipStream_t comp = nullptr, down = nullptr;
   std::vector<hipEvent_t> ev;
   ~PanelStreams() {
+    if (on_main_thread()) return;  // left to the process's end
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     if (comp) {
       release_workspace(comp);

@@ -22,12 +22,10 @@ int g_device = 0;
 
 struct ThreadState {
   hipStream_t stream = nullptr;
-  // A thread that ends releases its stream and workspaces.  The main
-  // thread's state goes with the process: its destructor runs while the
-  // process tears down, when the runtime (or a tool layered on it) may
-  // already be gone, so it makes no HIP calls.
+  // A thread that ends releases its stream and workspaces; the main
+  // thread's go with the process (on_main_thread).
   ~ThreadState() {
-    if (stream && syscall(SYS_gettid) != getpid()) {
+    if (stream && !on_main_thread()) {
       release_workspace(stream);
       (void)hipStreamDestroy(stream);
     }
@@ -68,6 +66,8 @@ void init_once() {
   g_init_status = FERRAY_OK;
 }
 }  // namespace
+
+bool on_main_thread() { return syscall(SYS_gettid) == getpid(); }
 
 int ensure_device() {
   std::call_once(g_init_once, init_once);

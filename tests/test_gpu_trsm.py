@@ -127,3 +127,27 @@ def test_invalid_strides_rejected(gpu):
                                                    False)
     with pytest.raises(ValueError):
         fa.trsm_left_lower_f64(4, 2, 1.0, a, 3, b, 2, False)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("lower,m,n", [(True, 1000, 4160), (False, 700, 4096), (True, 1300, 4224)])
+def test_columns_of_wide_solves_equal_narrow_solves(gpu, lower, dt, m, n):
+    """The columns of B are independent systems: every column of a wide
+    solve equals the same column solved in a narrower call, bit for bit (the
+    kernels fix each element's operation order whatever the column count),
+    and the whole meets the oracle's bound."""
+    a, b = _system(lower, m, n, dt, 7)
+    x = b.ravel().copy()
+    _gpu(lower, m, n, 1.0, a.ravel(), m, x, n, False)
+    X = x.reshape(m, n)
+    # 2000-column slices covering both ends and the middle
+    for c0 in (0, n // 2 - 1000, n - 2000):
+        part = np.ascontiguousarray(b[:, c0:c0 + 2000]).ravel()
+        _gpu(lower, m, 2000, 1.0, a.ravel(), m, part, 2000, False)
+        assert np.array_equal(part.reshape(m, 2000).view(np.uint8),
+                              np.ascontiguousarray(X[:, c0:c0 + 2000]).view(np.uint8)), c0
+    x_orc = b.ravel().copy()
+    oracle.trsm(lower, m, n, dt(1.0), a.ravel(), m, x_orc, n, False)
+    tol = 1e-10 if dt == np.float64 else 2e-4
+    assert np.linalg.norm(X.astype(np.float64) - x_orc.reshape(m, n)) <= \
+        tol * np.linalg.norm(x_orc)
