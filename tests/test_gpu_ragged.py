@@ -95,3 +95,30 @@ def test_unaligned_operands_equal_aligned(dt):
     aligned = _dev_gemm(dt, m, n, k, alpha, a, b, beta, c0)
     shifted = _dev_gemm(dt, m, n, k, alpha, a, b, beta, c0, offset=1)
     assert np.array_equal(aligned, shifted)
+
+
+# Products that leave CUs idle on 128 x 128 tiles run on 64 x 64 or, below
+# 192 of those, 32 x 32 tiles; every form accumulates each element over the
+# same MFMA k-sequence.  The same rows of A stacked under 8192 - m more rows
+# make a product on the 128 x 128 kernel, whose first m rows must match.
+@pytest.mark.parametrize("dt,m,n,k", [
+    (np.float64, 512, 512, 512), (np.float64, 768, 768, 768), (np.float64, 300, 200, 100),
+    (np.float64, 900, 896, 900), (np.float32, 512, 512, 512), (np.float32, 130, 1024, 77),
+    (np.complex128, 384, 384, 384), (np.complex64, 256, 512, 256)])
+def test_small_tile_forms_equal_the_large_kernel(gpu, dt, m, n, k):
+    r = rng(m * 7 + n)
+    cplx = np.dtype(dt).kind == "c"
+
+    def draw(*shape):
+        x = r.standard_normal(shape)
+        if cplx:
+            x = x + 1j * r.standard_normal(shape)
+        return x.astype(dt)
+
+    a, b = draw(m, k), draw(k, n)
+    big = 8192
+    a_big = np.concatenate([a, draw(big - m, k)])
+    one, zero = dt(1), dt(0)
+    c_small = _dev_gemm(dt, m, n, k, one, a, b, zero, np.zeros((m, n), dt))
+    c_big = _dev_gemm(dt, big, n, k, one, a_big, b, zero, np.zeros((big, n), dt))
+    assert np.array_equal(c_small.view(np.uint8), c_big[:m].view(np.uint8))
